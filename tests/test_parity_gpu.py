@@ -1,0 +1,201 @@
+"""Parity of the HIP engine with the CPU oracle, through the C ABI.
+
+FP64 with MEX semantics must be bitwise equal to the restated CPU path of
+BiotSavartMex (BiotSavart_nogpu, BiotSavartMex.cu:268-282): same operation
+order, same sequential sum, correctly rounded sqrt and division on both
+sides.  FP64 with BiotSavart.m semantics must agree within 1e-12 relative.
+FP32 compensated must agree within the bounds written in each test.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import flush_print
+from errstats import stats
+from oracle import bs as O
+import winds
+from winds import gen, layout
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1(need_gpu):
+    return gen.config1()
+
+
+def calls(w):
+    return {"shed": w.shed_segments(), "trail": w.trail_segments()}
+
+
+# --------------------------------------------------------------------------
+# FP64
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("type", ["fuln", "fuls", "full"])
+def test_fp64_mex_bitexact_config1(c1, type):
+    for name, (F1, F2, g, rc) in calls(c1).items():
+        flush_print(f"[fp64 mex {type}] {name}")
+        u, _ = winds.BiotSavartMex(F1, F2, c1.domain, g, rc, 1e-4, "none", 1000.0, type, "true",
+                                   (256, 1, 0))
+        ref = O.bs_mex(F1, F2, c1.domain, g, 1e-4)
+        assert np.array_equal(u, ref), (name, np.max(np.abs(u - ref)))
+
+
+@pytest.mark.parametrize("d", [1e-4, 0.0, "visc1", "visc2", "visc3"])
+def test_fp64_matlab_config1(c1, d):
+    for name, (F1, F2, g, rc) in calls(c1).items():
+        flush_print(f"[fp64 matlab {d}] {name}")
+        u, L = winds.BiotSavart(F1, F2, c1.domain, g, rc, d, 1000.0, "full")
+        ref = O.bs_matlab(F1, F2, c1.domain, g, rc, d, co=1000.0)
+        s = stats(u, ref)
+        flush_print(f"   {s}  exact={np.array_equal(u, ref)}")
+        assert s["max_rel"] <= 1e-12 and s["normwise"] <= 1e-12
+        # 'full' returns the squared filament length (BiotSavart.m:71)
+        assert np.allclose(L, O.filament_length(F1, F2) ** 2, rtol=1e-15, atol=0)
+
+
+def test_fp64_matlab_cutoff_bitexact(c1):
+    F1, F2, g, rc = c1.trail_segments()
+    u, _ = winds.BiotSavart(F1, F2, c1.domain, g, rc, 1e-4, 1000.0, "full")
+    ref = O.bs_matlab(F1, F2, c1.domain, g, rc, 1e-4, co=1000.0)
+    assert np.array_equal(u, ref)
+
+
+# --------------------------------------------------------------------------
+# FP32 compensated
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("cmod,d", [("none", 1e-4), ("cutm", 1e-4), ("visc", 1), ("visc", 2),
+                                    ("visc", 3)])
+def test_fp32c_config1(c1, cmod, d):
+    for name, (F1, F2, g, rc) in calls(c1).items():
+        flush_print(f"[fp32c {cmod}{d}] {name}")
+        u, _ = winds.BiotSavartMex(F1, F2, c1.domain, g, rc, d, cmod, 1000.0, "fulc", "true",
+                                   (256, 1, 0))
+        if cmod == "none":
+            ref = O.bs_mex(F1, F2, c1.domain, g, d)
+        else:
+            ref = O.bs_matlab(F1, F2, c1.domain, g, rc, d if cmod == "cutm" else f"visc{d}",
+                              co=1000.0)
+        s = stats(u, ref)
+        flush_print(f"   {s}")
+        # FP32 pair arithmetic, compensated sums, FP64 near pairs
+        assert s["max_rel"] <= 2e-5 and s["rms_rel"] <= 1e-6 and s["normwise"] <= 1e-6
+
+
+# --------------------------------------------------------------------------
+# edge cases of the boundary
+# --------------------------------------------------------------------------
+
+def seg(a, b):
+    return (np.asarray(a, float).reshape(1, 3, 1, 1, order="F"),
+            np.asarray(b, float).reshape(1, 3, 1, 1, order="F"))
+
+
+@pytest.mark.parametrize("type", ["fuln", "fulc"])
+def test_empty_inputs(need_gpu, type):
+    P = np.ones((4, 3, 2, 1), order="F")
+    F = np.zeros((0, 3, 5, 1), order="F")
+    u, L = winds.BiotSavartMex(F, F, P, np.zeros(0), np.zeros(0), 1e-4, "none", 1.0, type)
+    assert u.shape == (4, 3, 2, 1) and np.all(u == 0)
+    F1, F2 = seg([0, 0, 0], [1, 0, 0])
+    u, L = winds.BiotSavartMex(F1, F2, np.zeros((0, 3, 1, 1)), [1.0], [0.0], 1e-4, "none", 1.0, type)
+    assert u.shape == (0, 3, 1, 1)
+
+
+def test_ragged_target_shapes(c1):
+    F1, F2, g, rc = c1.shed_segments()
+    P4 = c1.domain[:, :, 5:6, 1:2]
+    P2 = np.asfortranarray(P4[:, :, 0, 0])          # 2-D P is padded to [n,3,1,1]
+    u2, _ = winds.BiotSavartMex(F1, F2, P2, g, rc, 1e-4, "none", 1.0, "fuln")
+    u4, _ = winds.BiotSavartMex(F1, F2, P4, g, rc, 1e-4, "none", 1.0, "fuln")
+    assert u2.shape == (P2.shape[0], 3, 1, 1)
+    assert np.array_equal(u2[:, :, 0, 0], u4[:, :, 0, 0])
+    one = np.asfortranarray(c1.domain[3:4, :, 7:8, 2:3])
+    u1, _ = winds.BiotSavartMex(F1, F2, one, g, rc, 1e-4, "none", 1.0, "fuln")
+    assert np.array_equal(u1, O.bs_mex(F1, F2, one, g, 1e-4))
+
+
+@pytest.mark.parametrize("type", ["fuln", "fulc"])
+def test_singular_targets(need_gpu, type):
+    F1, F2 = seg([0, 0, 0], [1, 0, 0])
+    P = np.asfortranarray(np.array([[0, 0, 0], [1, 0, 0], [2.5, 0, 0], [0.5, 0, 0],
+                                    [0.5, 0.3, 0]], float).reshape(5, 3, 1, 1))
+    g, rc = [5.0], [1e-3]
+    # MEX semantics, d > 0: exact zeros at endpoints and on the line
+    u, _ = winds.BiotSavartMex(F1, F2, P, g, rc, 1e-4, "none", 1.0, type)
+    ref = O.bs_mex(F1, F2, P, g, 1e-4)
+    assert np.all(u[:4] == 0) and np.all(ref[:4] == 0)
+    assert abs(u[4, 2, 0, 0] - ref[4, 2, 0, 0]) <= 1e-12 * abs(ref[4, 2, 0, 0])
+    # MEX semantics, d = 0: the reference's NaN at an endpoint is kept
+    u, _ = winds.BiotSavartMex(F1, F2, P[:1], g, rc, 0.0, "none", 1.0, type)
+    assert np.all(np.isnan(u))
+    # BiotSavart.m semantics: zeroed (BiotSavart.m:111)
+    for cmod, d in (("cutm", 0.0), ("visc", 1), ("visc", 2)):
+        u, _ = winds.BiotSavartMex(F1, F2, P, g, rc, d, cmod, 1000.0, type)
+        assert np.all(u[:4] == 0), (cmod, d)
+        assert np.isfinite(u).all()
+
+
+@pytest.mark.parametrize("type", ["fuln", "fulc"])
+def test_zero_length_segment_and_co(need_gpu, type):
+    F1 = np.asfortranarray(np.array([[0, 0, 0], [0, 0, 0]], float).reshape(2, 3, 1, 1))
+    F2 = np.asfortranarray(np.array([[0, 0, 0], [1, 0, 0]], float).reshape(2, 3, 1, 1))
+    P = np.asfortranarray(np.array([[0.5, 2.0, 0], [0.5, 50.0, 0], [0, 0, 0]], float)
+                          .reshape(3, 3, 1, 1))
+    g, rc = [3.0, 2.0], [1e-3, 1e-3]
+    for cmod, d in (("cutm", 1e-4), ("visc", 2)):
+        u, _ = winds.BiotSavartMex(F1, F2, P, g, rc, d, cmod, 10.0, type)
+        dd = d if cmod == "cutm" else f"visc{d}"
+        ref = O.bs_matlab(F1, F2, P, g, rc, dd, co=10.0)
+        assert np.isfinite(u).all()
+        assert u[1, 2, 0, 0] == 0.0 and ref[1, 2, 0, 0] == 0.0   # beyond co
+        assert np.all(u[2] == 0)
+        tol = 0 if type == "fuln" else 1e-6
+        assert abs(u[0, 2, 0, 0] - ref[0, 2, 0, 0]) <= tol * abs(ref[0, 2, 0, 0])
+
+
+@pytest.mark.parametrize("threads", [64, 128, 512, 1024, 0])
+def test_block_sizes_do_not_change_fp64_bits(c1, threads):
+    F1, F2, g, rc = c1.trail_segments()
+    u, _ = winds.BiotSavartMex(F1, F2, c1.domain, g, rc, 1e-4, "none", 1.0, "fuln", "true",
+                               (threads, 1, 0))
+    assert np.array_equal(u, O.bs_mex(F1, F2, c1.domain, g, 1e-4))
+
+
+def test_length_mode(c1):
+    F1, F2, g, rc = c1.trail_segments()
+    for t in ("leng", "length"):
+        u, L = winds.BiotSavartMex(F1, F2, c1.domain, g, rc, 1e-4, "none", 1.0, t)
+        assert np.all(u == 0)
+        assert np.array_equal(L, O.filament_length(F1, F2))
+    u, L = winds.BiotSavart(F1, F2, c1.domain, g, rc, 1e-4, 1000.0, "length")
+    assert np.array_equal(L, O.filament_length(F1, F2))
+
+
+# --------------------------------------------------------------------------
+# device-resident API
+# --------------------------------------------------------------------------
+
+def test_device_api_matches_host_api(c1):
+    import torch
+    from winds.device import Evaluator
+    ev = Evaluator(0)
+    F1, F2, g, rc = c1.trail_segments()
+    segs = torch.from_numpy(layout.segments(F1, F2, g, rc)).cuda()
+    tg = torch.from_numpy(layout.targets(c1.domain)).cuda().contiguous()
+    u = ev.eval(tg, segs, winds.opts(d=1e-4)).cpu().numpy()
+    ref = O.bs_mex(F1, F2, c1.domain, g, 1e-4)
+    assert np.array_equal(layout.uind_to_cm(u, c1.domain.shape), ref)
+    # FP32 compensated: the split of the segment range changes only rounding
+    outs = []
+    for split in (1, 3, 8):
+        o = winds.opts(precision=winds.PREC_FP32C, d=1e-4, split=split)
+        outs.append(ev.eval(tg, segs, o).cpu().numpy())
+        s = stats(layout.uind_to_cm(outs[-1], c1.domain.shape), ref)
+        flush_print(f"[split {split}] {s}")
+        assert s["normwise"] <= 1e-6
+    for x in outs[1:]:
+        assert np.max(np.abs(x - outs[0])) <= 1e-7 * np.max(np.abs(outs[0]))

@@ -1,0 +1,458 @@
+// bs_kernels.hip -- Biot-Savart direct-sum kernels for gfx950 (CDNA4).
+//
+// Work decomposition: one target per lane, every lane of a wavefront walks
+// the same segment sequence, so each segment record is wave-uniform and is
+// fetched with scalar loads into SGPRs (no LDS staging, no barriers); the
+// VALU only does per-pair arithmetic.  Segment-only terms (L, |L|^2,
+// 1/|L|^2, gamma/4pi, core constant, near radius) are hoisted into a
+// prepared record by prep_seg32.
+//
+// Kernels:
+//   bs_fp64_mex      FP64, biot_p2p arithmetic and the sequential segment
+//                    order of BiotSavart_nogpu (BiotSavartMex.cu:63-93,
+//                    268-282); bitwise equal to the CPU restatement.
+//   bs_fp64_matlab   FP64, BiotSavart.m arithmetic (:86-115) and its nested
+//                    sum order sum(sum(sum(.,1),3),4).
+//   bs_fp32c         FP32 pair arithmetic with r1 = P-A and r2 = P-B taken in
+//                    FP64 and rounded once, r1 x r2 evaluated as L x r1 (no
+//                    cancellation), per-tile partial sums merged into a
+//                    (hi, lo) two-sum accumulator; pairs whose target lies
+//                    within the near radius of the segment are evaluated in
+//                    FP64 with the reference arithmetic instead (inline
+//                    divergent branch, taken by few waves).
+//   bs_reduce_split  sums the per-split partials in a fixed order.
+//   pack/unpack      column-major MATLAB arrays <-> device layout, in the
+//                    reference's packed index k = m + n*ns + p*ns*nt.
+
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <stdint.h>
+
+#include "bs_kernels.hpp"
+#include "bs_pair.hpp"
+
+namespace winds {
+
+// ---------------------------------------------------------------------------
+// FP64 kernels
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(1024) void bs_fp64_mex(
+    const double *__restrict__ tx, const double *__restrict__ ty,
+    const double *__restrict__ tz, int64_t nt, const Seg64 *__restrict__ seg,
+    int64_t nseg, double d, double *__restrict__ ux, double *__restrict__ uy,
+    double *__restrict__ uz) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const double px = tx[t], py = ty[t], pz = tz[t];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int64_t k = 0; k < nseg; ++k) {
+        const Seg64 s = seg[k];
+        mex_pair(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, px, py, pz, d, sx, sy, sz);
+    }
+    ux[t] = sx;
+    uy[t] = sy;
+    uz[t] = sz;
+}
+
+// col = segments per station column, blk = segments per blade (col * ages).
+__global__ __launch_bounds__(1024) void bs_fp64_matlab(
+    const double *__restrict__ tx, const double *__restrict__ ty,
+    const double *__restrict__ tz, int64_t nt, const Seg64 *__restrict__ seg,
+    int64_t nseg, int64_t col, int64_t blk, double d, int n, double co,
+    double *__restrict__ ux, double *__restrict__ uy, double *__restrict__ uz) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const double px = tx[t], py = ty[t], pz = tz[t];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int64_t b0 = 0; b0 < nseg; b0 += blk) {
+        const int64_t b1 = b0 + blk < nseg ? b0 + blk : nseg;
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        for (int64_t c0 = b0; c0 < b1; c0 += col) {
+            const int64_t c1 = c0 + col < b1 ? c0 + col : b1;
+            double cx = 0.0, cy = 0.0, cz = 0.0;
+            for (int64_t k = c0; k < c1; ++k) {
+                const Seg64 s = seg[k];
+                double ex, ey, ez;
+                matlab_pair(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, s.rc, px, py, pz, d,
+                            n, co, ex, ey, ez);
+                cx += ex;
+                cy += ey;
+                cz += ez;
+            }
+            bx += cx;
+            by += cy;
+            bz += cz;
+        }
+        sx += bx;
+        sy += by;
+        sz += bz;
+    }
+    ux[t] = sx;
+    uy[t] = sy;
+    uz[t] = sz;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 compensated kernel
+// ---------------------------------------------------------------------------
+
+__global__ void prep_seg32(const Seg64 *__restrict__ seg, int64_t nseg,
+                           int64_t nseg_pad, int core_kind, int n, double d,
+                           double near_factor, double near_len_frac,
+                           Seg32 *__restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nseg_pad) return;
+    Seg32 r;
+    if (k >= nseg) {
+        // Padding record: far away, zero strength, never near.  Every FP32
+        // intermediate stays finite, so its term is exactly zero.
+        r.ax = r.ay = r.az = 1.0e6;
+        r.bx = r.by = r.bz = 1.0e6;
+        r.lx = r.ly = r.lz = 0.0f;
+        r.l2 = 0.0f;
+        r.il2 = 0.0f;
+        r.g = 0.0f;
+        r.core = 1.0f;
+        r.near2 = -1.0f;
+        out[k] = r;
+        return;
+    }
+    const Seg64 s = seg[k];
+    const double lx = s.bx - s.ax, ly = s.by - s.ay, lz = s.bz - s.az;
+    const double l2 = lx * lx + ly * ly + lz * lz;
+    r.ax = s.ax;
+    r.ay = s.ay;
+    r.az = s.az;
+    r.bx = s.bx;
+    r.by = s.by;
+    r.bz = s.bz;
+    r.lx = (float)lx;
+    r.ly = (float)ly;
+    r.lz = (float)lz;
+    r.l2 = (float)l2;
+    r.il2 = l2 > 0.0 ? (float)(1.0 / l2) : 0.0f;
+    r.g = l2 > 0.0 ? (float)(s.g / (4.0 * kMlPi)) : 0.0f;
+    double core;
+    if (core_kind == 0) {
+        core = d * l2;
+    } else {
+        core = ipow(s.rc, n);
+    }
+    r.core = (float)core;
+    double nr2 = near_factor * near_factor * s.rc;
+    const double nl2 = near_len_frac * near_len_frac * l2;
+    if (nl2 > nr2) nr2 = nl2;
+    if (!(nr2 > 0.0) || !(s.g == s.g)) nr2 = __builtin_inf();
+    r.near2 = (float)nr2;
+    out[k] = r;
+}
+
+__device__ __forceinline__ void two_sum_acc(float &hi, float &lo, float v) {
+    const float s = hi + v;
+    const float bb = s - hi;
+    const float err = (hi - (s - bb)) + (v - bb);
+    hi = s;
+    lo += err;
+}
+
+// CORE: 0 cut-off, 1 Vatistas n=1, 2 Vatistas n=2, 3 Vatistas general n.
+// SEM:  0 MEX semantics (near pairs by biot_p2p), 1 BiotSavart.m semantics
+//       (co cut-off; near pairs by the BiotSavart.m arithmetic with zeroing).
+template <int CORE, int SEM>
+__global__ __launch_bounds__(256) void bs_fp32c(
+    const double *__restrict__ tx, const double *__restrict__ ty,
+    const double *__restrict__ tz, int64_t nt, const Seg32 *__restrict__ s32,
+    const Seg64 *__restrict__ s64, int64_t nseg_pad, int64_t chunk, float co2,
+    double d, int vat_n, double co, float neg_inv_n, double *__restrict__ out,
+    int64_t out_stride) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int64_t k0 = (int64_t)blockIdx.y * chunk;
+    const int64_t k1 = k0 + chunk < nseg_pad ? k0 + chunk : nseg_pad;
+    const double px = tx[t], py = ty[t], pz = tz[t];
+    float hx = 0.f, hy = 0.f, hz = 0.f, lx = 0.f, ly = 0.f, lz = 0.f;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int64_t kt = k0; kt < k1; kt += kSegTile) {
+        float qx = 0.f, qy = 0.f, qz = 0.f;
+#pragma unroll 2
+        for (int j = 0; j < kSegTile; ++j) {
+            const int64_t k = kt + j;
+            const Seg32 s = s32[k];
+            const float r1x = (float)(px - s.ax), r1y = (float)(py - s.ay),
+                        r1z = (float)(pz - s.az);
+            const float r2x = (float)(px - s.bx), r2y = (float)(py - s.by),
+                        r2z = (float)(pz - s.bz);
+            const float s1 = fmaf(r1z, r1z, fmaf(r1y, r1y, r1x * r1x));
+            const float s2 = fmaf(r2z, r2z, fmaf(r2y, r2y, r2x * r2x));
+            const float dot = fmaf(r1z, r2z, fmaf(r1y, r2y, r1x * r2x));
+            // r1 x r2 = L x r1 with L = B - A.
+            const float cx = fmaf(s.ly, r1z, -(s.lz * r1y));
+            const float cy = fmaf(s.lz, r1x, -(s.lx * r1z));
+            const float cz = fmaf(s.lx, r1y, -(s.ly * r1x));
+            const float tp = s1 - dot;  // r1 . L
+            // squared distance from the target to the closed segment
+            const float tc = __builtin_amdgcn_fmed3f(tp, 0.0f, s.l2);
+            const float d2 = fmaf(-(tc * fmaf(2.0f, tp, -tc)), s.il2, s1);
+            const bool near = !(d2 >= s.near2);
+            const float i1 = __builtin_amdgcn_rsqf(s1);
+            const float i2 = __builtin_amdgcn_rsqf(s2);
+            const float pp = i1 * i2;
+            const float r1r2 = (s1 * s2) * pp;
+            float den = r1r2 + dot;
+            const float num = i1 + i2;
+            float ubar;
+            if constexpr (CORE == 0) {
+                den = fmaf(s.core, pp, den);
+                ubar = (s.g * num) * __builtin_amdgcn_rcpf(den);
+            } else if constexpr (CORE == 1) {
+                const float h2 = fmaf(-(tp * tp), s.il2, s1);
+                ubar = (s.g * num) * (h2 * __builtin_amdgcn_rcpf((s.core + h2) * den));
+            } else if constexpr (CORE == 2) {
+                const float h2 = fmaf(-(tp * tp), s.il2, s1);
+                const float y = fmaxf(fmaf(h2, h2, s.core), FLT_MIN) * (den * den);
+                ubar = (s.g * num) * (h2 * __builtin_amdgcn_rsqf(y));
+            } else {
+                const float h2 = fmaf(-(tp * tp), s.il2, s1);
+                float hn = h2;
+                for (int i = 1; i < vat_n; ++i) hn = hn * h2;
+                const float y = fmaxf(s.core + hn, FLT_MIN);
+                const float c = h2 * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(y) * neg_inv_n);
+                ubar = (s.g * num) * (c * __builtin_amdgcn_rcpf(den));
+            }
+            bool zero = near;
+            if constexpr (SEM == 1) zero = zero || (s1 > co2 && s2 > co2);
+            ubar = zero ? 0.0f : ubar;
+            qx = fmaf(ubar, cx, qx);
+            qy = fmaf(ubar, cy, qy);
+            qz = fmaf(ubar, cz, qz);
+            if (near) {
+                const Seg64 q = s64[k];
+                if constexpr (SEM == 0) {
+                    mex_pair(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, px, py, pz, d, ax, ay, az);
+                } else {
+                    double ex, ey, ez;
+                    matlab_pair(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, q.rc, px, py, pz, d,
+                                CORE == 0 ? 0 : vat_n, co, ex, ey, ez);
+                    ax += ex;
+                    ay += ey;
+                    az += ez;
+                }
+            }
+        }
+        two_sum_acc(hx, lx, qx);
+        two_sum_acc(hy, ly, qy);
+        two_sum_acc(hz, lz, qz);
+    }
+    double *o = out + (int64_t)blockIdx.y * 3 * out_stride;
+    o[t] = ((double)hx + (double)lx) + ax;
+    o[out_stride + t] = ((double)hy + (double)ly) + ay;
+    o[2 * out_stride + t] = ((double)hz + (double)lz) + az;
+}
+
+__global__ void bs_reduce_split(const double *__restrict__ part, int64_t nt,
+                                int split, double *__restrict__ ux,
+                                double *__restrict__ uy, double *__restrict__ uz) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int s = 0; s < split; ++s) {
+        const double *p = part + (int64_t)s * 3 * nt;
+        sx += p[t];
+        sy += p[nt + t];
+        sz += p[2 * nt + t];
+    }
+    ux[t] = sx;
+    uy[t] = sy;
+    uz[t] = sz;
+}
+
+// ---------------------------------------------------------------------------
+// Layout kernels
+// ---------------------------------------------------------------------------
+
+__global__ void pack_segments_cm(const double *__restrict__ F1,
+                                 const double *__restrict__ F2, int64_t ns,
+                                 int64_t ntm, int64_t nb,
+                                 const double *__restrict__ gamma,
+                                 const double *__restrict__ rc,
+                                 Seg64 *__restrict__ out) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nf = ns * ntm * nb;
+    if (k >= nf) return;
+    const int64_t m = k % ns, n = (k / ns) % ntm, p = k / (ns * ntm);
+    const int64_t base = m + 3 * ns * n + 3 * ns * ntm * p;
+    Seg64 s;
+    s.ax = F1[base];
+    s.ay = F1[base + ns];
+    s.az = F1[base + 2 * ns];
+    s.bx = F2[base];
+    s.by = F2[base + ns];
+    s.bz = F2[base + 2 * ns];
+    s.g = gamma[k];
+    s.rc = rc ? rc[k] : 0.0;
+    out[k] = s;
+}
+
+__global__ void pack_targets_cm(const double *__restrict__ P, int64_t ns,
+                                int64_t ntm, int64_t nb, double *__restrict__ t3) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np = ns * ntm * nb;
+    if (j >= np) return;
+    const int64_t m = j % ns, n = (j / ns) % ntm, p = j / (ns * ntm);
+    const int64_t base = m + 3 * ns * n + 3 * ns * ntm * p;
+    t3[j] = P[base];
+    t3[np + j] = P[base + ns];
+    t3[2 * np + j] = P[base + 2 * ns];
+}
+
+__global__ void unpack_uind_cm(const double *__restrict__ u3, int64_t ns,
+                               int64_t ntm, int64_t nb, double *__restrict__ U) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t np = ns * ntm * nb;
+    if (j >= np) return;
+    const int64_t m = j % ns, n = (j / ns) % ntm, p = j / (ns * ntm);
+    const int64_t base = m + 3 * ns * n + 3 * ns * ntm * p;
+    U[base] = u3[j];
+    U[base + ns] = u3[np + j];
+    U[base + 2 * ns] = u3[2 * np + j];
+}
+
+__global__ void segment_length(const Seg64 *__restrict__ seg, int64_t nseg,
+                               int squared, double *__restrict__ L) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nseg) return;
+    const Seg64 s = seg[k];
+    const double x2x1 = s.bx - s.ax, y2y1 = s.by - s.ay, z2z1 = s.bz - s.az;
+    const double l2 = x2x1 * x2x1 + y2y1 * y2y1 + z2z1 * z2z1;
+    L[k] = squared ? l2 : sqrt(l2);
+}
+
+__global__ void fill_zero(double *__restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+static inline unsigned grid_for(int64_t n, int block) {
+    return (unsigned)((n + block - 1) / block);
+}
+
+hipError_t launch_zero(double *p, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    fill_zero<<<grid_for(n, 256), 256, 0, st>>>(p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fp64(const double *t3, int64_t nt, const Seg64 *seg,
+                       int64_t nseg, const EvalCfg &c, double *u3, hipStream_t st) {
+    const int block = c.block > 0 ? c.block : 256;
+    if (nt <= 0) return hipSuccess;
+    if (nseg <= 0) return launch_zero(u3, 3 * nt, st);
+    const double *tx = t3, *ty = t3 + nt, *tz = t3 + 2 * nt;
+    double *ux = u3, *uy = u3 + nt, *uz = u3 + 2 * nt;
+    if (c.semantics == 0) {
+        bs_fp64_mex<<<grid_for(nt, block), block, 0, st>>>(tx, ty, tz, nt, seg, nseg, c.d, ux, uy, uz);
+    } else {
+        const int64_t col = c.seg_col > 0 ? c.seg_col : nseg;
+        const int64_t blk = c.seg_blk > 0 ? c.seg_blk : nseg;
+        const int n = c.core == 0 ? 0 : c.vat_n;
+        bs_fp64_matlab<<<grid_for(nt, block), block, 0, st>>>(tx, ty, tz, nt, seg, nseg, col, blk,
+                                                              c.d, n, c.co, ux, uy, uz);
+    }
+    return hipGetLastError();
+}
+
+int64_t seg32_padded(int64_t nseg) { return (nseg + kSegTile - 1) / kSegTile * kSegTile; }
+
+template <int CORE, int SEM>
+static void fp32c_dispatch(dim3 grid, int block, hipStream_t st, const double *tx,
+                           const double *ty, const double *tz, int64_t nt,
+                           const Seg32 *s32, const Seg64 *s64, int64_t npad,
+                           int64_t chunk, float co2, double d, int n, double co,
+                           float ninv, double *out, int64_t stride) {
+    bs_fp32c<CORE, SEM><<<grid, block, 0, st>>>(tx, ty, tz, nt, s32, s64, npad, chunk, co2, d, n,
+                                                co, ninv, out, stride);
+}
+
+hipError_t launch_fp32c(const double *t3, int64_t nt, const Seg64 *seg,
+                        int64_t nseg, const EvalCfg &c, double *u3, Seg32 *s32,
+                        double *part, hipStream_t st) {
+    if (nt <= 0) return hipSuccess;
+    if (nseg <= 0) return launch_zero(u3, 3 * nt, st);
+    const int64_t npad = seg32_padded(nseg);
+    const int core_kind = c.core == 0 ? 0 : 1;
+    const int n = c.core == 0 ? 0 : c.vat_n;
+    prep_seg32<<<grid_for(npad, 256), 256, 0, st>>>(seg, nseg, npad, core_kind, n, c.d,
+                                                    c.near_factor, c.near_len_frac, s32);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int block = c.block > 0 ? c.block : 256;
+    const int split = c.split > 0 ? c.split : 1;
+    const int64_t tiles = npad / kSegTile;
+    const int64_t chunk = (tiles + split - 1) / split * kSegTile;
+    const int nsplit = (int)((npad + chunk - 1) / chunk);
+    dim3 grid(grid_for(nt, block), nsplit);
+    const double *tx = t3, *ty = t3 + nt, *tz = t3 + 2 * nt;
+    double *out = nsplit == 1 ? u3 : part;
+    const double cof = c.co;
+    const float co2 = (c.semantics == 1 && cof < 1.0e18) ? (float)(cof * cof) : __builtin_inff();
+    const float ninv = n > 0 ? -1.0f / (float)n : 0.0f;
+    const int kind = c.core == 0 ? 0 : (n == 1 ? 1 : (n == 2 ? 2 : 3));
+#define WINDS_FP32C(K, S)                                                                   \
+    fp32c_dispatch<K, S>(grid, block, st, tx, ty, tz, nt, s32, seg, npad, chunk, co2, c.d, n, \
+                         c.co, ninv, out, nt)
+    if (c.semantics == 0) {
+        WINDS_FP32C(0, 0);
+    } else {
+        switch (kind) {
+            case 0: WINDS_FP32C(0, 1); break;
+            case 1: WINDS_FP32C(1, 1); break;
+            case 2: WINDS_FP32C(2, 1); break;
+            default: WINDS_FP32C(3, 1); break;
+        }
+    }
+#undef WINDS_FP32C
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (nsplit > 1) {
+        bs_reduce_split<<<grid_for(nt, 256), 256, 0, st>>>(part, nt, nsplit, u3, u3 + nt, u3 + 2 * nt);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+hipError_t launch_pack_segments(const double *F1, const double *F2, const int64_t *fd,
+                                const double *gamma, const double *rc, Seg64 *out,
+                                hipStream_t st) {
+    const int64_t nf = fd[0] * fd[2] * fd[3];
+    if (nf <= 0) return hipSuccess;
+    pack_segments_cm<<<grid_for(nf, 256), 256, 0, st>>>(F1, F2, fd[0], fd[2], fd[3], gamma, rc, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_targets(const double *P, const int64_t *pd, double *t3, hipStream_t st) {
+    const int64_t np = pd[0] * pd[2] * pd[3];
+    if (np <= 0) return hipSuccess;
+    pack_targets_cm<<<grid_for(np, 256), 256, 0, st>>>(P, pd[0], pd[2], pd[3], t3);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_uind(const double *u3, const int64_t *pd, double *U, hipStream_t st) {
+    const int64_t np = pd[0] * pd[2] * pd[3];
+    if (np <= 0) return hipSuccess;
+    unpack_uind_cm<<<grid_for(np, 256), 256, 0, st>>>(u3, pd[0], pd[2], pd[3], U);
+    return hipGetLastError();
+}
+
+hipError_t launch_segment_length(const Seg64 *seg, int64_t nseg, int squared, double *L,
+                                 hipStream_t st) {
+    if (nseg <= 0) return hipSuccess;
+    segment_length<<<grid_for(nseg, 256), 256, 0, st>>>(seg, nseg, squared, L);
+    return hipGetLastError();
+}
+
+}  // namespace winds

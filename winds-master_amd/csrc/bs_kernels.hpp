@@ -1,0 +1,62 @@
+// bs_kernels.hpp -- host-side view of the direct-sum kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bs_pair.hpp"
+
+namespace winds {
+
+// Segments per compensated partial sum in the FP32 kernel; segment tables
+// are padded to a multiple of it.
+constexpr int kSegTile = 32;
+
+// Prepared segment record of the FP32 kernel (80 bytes, read with scalar
+// loads).  The endpoints stay FP64 so that r1 = P-A and r2 = P-B are formed
+// in FP64 and rounded once; everything else is segment-only FP32.
+struct __attribute__((aligned(16))) Seg32 {
+    double ax, ay, az;   // F1 endpoint
+    double bx, by, bz;   // F2 endpoint
+    float lx, ly, lz;    // L = B - A, rounded once
+    float l2;            // |L|^2
+    float il2;           // 1/|L|^2 (0 for a zero-length segment)
+    float g;             // gamma / (4 pi) (0 for a zero-length segment)
+    float core;          // cut-off: d |L|^2; Vatistas: rc^n (rc = r_c^2)
+    float near2;         // squared near radius; +inf: always FP64, <0: never
+};
+static_assert(sizeof(Seg32) == 80, "Seg32 layout");
+static_assert(sizeof(Seg64) == 64, "Seg64 layout");
+
+struct EvalCfg {
+    int core = 0;        // 0 cut-off, 1 Vatistas
+    int vat_n = 2;
+    int semantics = 0;   // 0 MEX, 1 MATLAB
+    int precision = 0;   // 0 FP64, 1 FP32C
+    double d = 1e-4;
+    double co = __builtin_inf();
+    double near_factor = 5.0;
+    double near_len_frac = 1.0 / 16.0;
+    int block = 0;
+    int split = 0;
+    int64_t seg_col = 0;
+    int64_t seg_blk = 0;
+};
+
+int64_t seg32_padded(int64_t nseg);
+
+hipError_t launch_zero(double *p, int64_t n, hipStream_t st);
+hipError_t launch_fp64(const double *t3, int64_t nt, const Seg64 *seg, int64_t nseg,
+                       const EvalCfg &c, double *u3, hipStream_t st);
+hipError_t launch_fp32c(const double *t3, int64_t nt, const Seg64 *seg, int64_t nseg,
+                        const EvalCfg &c, double *u3, Seg32 *s32, double *part,
+                        hipStream_t st);
+hipError_t launch_pack_segments(const double *F1, const double *F2, const int64_t *fd,
+                                const double *gamma, const double *rc, Seg64 *out,
+                                hipStream_t st);
+hipError_t launch_pack_targets(const double *P, const int64_t *pd, double *t3, hipStream_t st);
+hipError_t launch_unpack_uind(const double *u3, const int64_t *pd, double *U, hipStream_t st);
+hipError_t launch_segment_length(const Seg64 *seg, int64_t nseg, int squared, double *L,
+                                 hipStream_t st);
+
+}  // namespace winds

@@ -1,0 +1,99 @@
+// bs_pair.hpp -- one target/segment interaction in FP64, in the reference's
+// own operation order.  Shared by the FP64 kernels and by the near-pair
+// branch of the FP32 compensated kernel, so both round exactly alike.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace winds {
+
+// Segment record in device memory: A = F1 endpoint, B = F2 endpoint,
+// gamma, rc = squared core radius.  64 bytes, one scalar-cache line.
+struct __attribute__((aligned(64))) Seg64 {
+    double ax, ay, az, bx, by, bz, g, rc;
+};
+
+// BiotSavartMex.cu:35 spells pi as this literal.
+constexpr double kMexPi = 3.14159265358979324;
+// MATLAB's pi (the double nearest pi; the same value).
+constexpr double kMlPi = 3.141592653589793;
+
+// biot_p2p / biot_p2p_nogpu (BiotSavartMex.cu:63-93, 236-266): cut-off core
+// with d*|L|^2 in the denominator; rc and co unused; nothing zeroed.
+// x1,y1,z1 = F1 endpoint, x2,y2,z2 = F2 endpoint (the reference packs them
+// as fx = (x1, x2), BiotSavartMex.cu:347-352).
+__device__ __forceinline__ void mex_pair(double x1, double y1, double z1,
+                                         double x2, double y2, double z2,
+                                         double g, double px, double py,
+                                         double pz, double cd, double &ux,
+                                         double &uy, double &uz) {
+    double ldx = x1 - x2;
+    double ldy = y1 - y2;
+    double ldz = z1 - z2;
+    double pxx1 = px - x1;
+    double pxx2 = px - x2;
+    double pyy1 = py - y1;
+    double pyy2 = py - y2;
+    double pzz1 = pz - z1;
+    double pzz2 = pz - z2;
+    double r1 = sqrt(pxx1 * pxx1 + pyy1 * pyy1 + pzz1 * pzz1);
+    double r2 = sqrt(pxx2 * pxx2 + pyy2 * pyy2 + pzz2 * pzz2);
+    double r1dr2 = pxx1 * pxx2 + pyy1 * pyy2 + pzz1 * pzz2;
+    double r1tr2 = r1 * r2;
+    double len = ldx * ldx + ldy * ldy + ldz * ldz;
+    double den = r1tr2 * (r1tr2 + r1dr2) + cd * len;
+    double ubar = (g * (r1 + r2)) * (1.000 / (4.000 * kMexPi));
+    ubar = ubar / den;
+    ux += ubar * (pyy1 * pzz2 - pzz1 * pyy2);
+    uy += ubar * (pzz1 * pxx2 - pxx1 * pzz2);
+    uz += ubar * (pxx1 * pyy2 - pyy1 * pxx2);
+}
+
+__device__ __forceinline__ double ipow(double x, int n) {
+    double r = x;
+    for (int i = 1; i < n; ++i) r = r * x;
+    return r;
+}
+
+// y.^(-1/n): 1/y for n = 1, 1/sqrt(y) for n = 2, pow otherwise.
+__device__ __forceinline__ double inv_root(double y, int n) {
+    if (n == 1) return 1.0 / y;
+    if (n == 2) return 1.0 / sqrt(y);
+    return pow(y, -1.0 / (double)n);
+}
+
+// BiotSavart.m:86-115 for one pair: cut-off core (n == 0) or Vatistas
+// order n (n >= 1, rc = r_c^2), each expression evaluated left to right as
+// MATLAB does, ubar zeroed when NaN/Inf or when r1 > co and r2 > co
+// (BiotSavart.m:111).  Returns the term ubar * (r1 x r2).
+__device__ __forceinline__ void matlab_pair(double x1, double y1, double z1,
+                                            double x2, double y2, double z2,
+                                            double g, double rc, double px,
+                                            double py, double pz, double d,
+                                            int n, double co, double &tx,
+                                            double &ty, double &tz) {
+    double x2x1 = x2 - x1, y2y1 = y2 - y1, z2z1 = z2 - z1;
+    double L = x2x1 * x2x1 + y2y1 * y2y1 + z2z1 * z2z1;
+    double pxx1 = px - x1, pyy1 = py - y1, pzz1 = pz - z1;
+    double pxx2 = px - x2, pyy2 = py - y2, pzz2 = pz - z2;
+    double r1 = sqrt(pxx1 * pxx1 + pyy1 * pyy1 + pzz1 * pzz1);
+    double r2 = sqrt(pxx2 * pxx2 + pyy2 * pyy2 + pzz2 * pzz2);
+    double r1dr2 = pxx1 * pxx2 + pyy1 * pyy2 + pzz1 * pzz2;
+    double r1tr2 = r1 * r2;
+    double ubar;
+    if (n != 0) {
+        double ld = x2x1 * pxx1 + y2y1 * pyy1 + z2z1 * pzz1;
+        double Ldr12 = ld * ld;
+        double Cnu = (r1 * r1) - (Ldr12 / L);
+        Cnu = Cnu * inv_root(ipow(rc, n) + ipow(Cnu, n), n);
+        ubar = Cnu * g / (4.0 * kMlPi) * (r1 + r2) / (r1tr2 * (r1tr2 + r1dr2));
+    } else {
+        ubar = (g / (4.0 * kMlPi)) * (r1 + r2) / (r1tr2 * (r1tr2 + r1dr2) + (d * L));
+    }
+    if (__builtin_isnan(ubar) || __builtin_isinf(ubar) || (r1 > co && r2 > co)) ubar = 0.0;
+    tx = ubar * (pyy1 * pzz2 - pzz1 * pyy2);
+    ty = ubar * (pzz1 * pxx2 - pxx1 * pzz2);
+    tz = ubar * (pxx1 * pyy2 - pyy1 * pxx2);
+}
+
+}  // namespace winds
