@@ -59,7 +59,7 @@ def cpu_baseline(wake, budget_s: float = 12.0) -> dict:
     threads = host_cores()
     out = {}
     for label, th, frac in (("all", threads, 0.75), ("one", 1, 0.25)):
-        n = 64 if th > 1 else 16
+        n = 16 * th if th > 1 else 16
         t0 = time.perf_counter()
         O.bs_mex_range(F1, F2, P, g, 1e-4, 0, n, threads=th)
         dt = time.perf_counter() - t0

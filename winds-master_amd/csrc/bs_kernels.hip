@@ -190,6 +190,7 @@ __global__ __launch_bounds__(256) void bs_fp32c(
             const float cx = fmaf(s.ly, r1z, -(s.lz * r1y));
             const float cy = fmaf(s.lz, r1x, -(s.lx * r1z));
             const float cz = fmaf(s.lx, r1y, -(s.ly * r1x));
+            const float c2 = fmaf(cz, cz, fmaf(cy, cy, cx * cx));  // |r1 x r2|^2
             const float tp = s1 - dot;  // r1 . L
             // squared distance from the target to the closed segment
             const float tc = __builtin_amdgcn_fmed3f(tp, 0.0f, s.l2);
@@ -199,26 +200,40 @@ __global__ __launch_bounds__(256) void bs_fp32c(
             const float i2 = __builtin_amdgcn_rsqf(s2);
             const float pp = i1 * i2;
             const float r1r2 = (s1 * s2) * pp;
-            float den = r1r2 + dot;
+            // r1 r2 + r1.r2 cancels when the target sits beside the segment
+            // (r1.r2 < 0); there it equals |r1 x r2|^2 / (r1 r2 - r1.r2),
+            // whose terms are both accurate.  qa = r1 r2 + |r1.r2| never cancels.
+            const float qa = r1r2 + fabsf(dot);
+            const bool side = dot <= 0.0f;
             const float num = i1 + i2;
             float ubar;
             if constexpr (CORE == 0) {
-                den = fmaf(s.core, pp, den);
-                ubar = (s.g * num) * __builtin_amdgcn_rcpf(den);
-            } else if constexpr (CORE == 1) {
-                const float h2 = fmaf(-(tp * tp), s.il2, s1);
-                ubar = (s.g * num) * (h2 * __builtin_amdgcn_rcpf((s.core + h2) * den));
-            } else if constexpr (CORE == 2) {
-                const float h2 = fmaf(-(tp * tp), s.il2, s1);
-                const float y = fmaxf(fmaf(h2, h2, s.core), FLT_MIN) * (den * den);
-                ubar = (s.g * num) * (h2 * __builtin_amdgcn_rsqf(y));
+                // 1/(r1r2 + dot + d|L|^2/(r1r2)), in the form matching side
+                const float t1 = s.core * pp;
+                const float nume = side ? qa : 1.0f;
+                const float deno = side ? fmaf(t1, qa, c2) : (qa + t1);
+                ubar = ((s.g * num) * nume) * __builtin_amdgcn_rcpf(deno);
             } else {
-                const float h2 = fmaf(-(tp * tp), s.il2, s1);
-                float hn = h2;
-                for (int i = 1; i < vat_n; ++i) hn = hn * h2;
-                const float y = fmaxf(s.core + hn, FLT_MIN);
-                const float c = h2 * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(y) * neg_inv_n);
-                ubar = (s.g * num) * (c * __builtin_amdgcn_rcpf(den));
+                const float h2 = c2 * s.il2;  // squared distance to the line, >= 0
+                // C/den with C the Vatistas factor: side: il2 qa C/h2,
+                // otherwise C/qa
+                const float F = side ? s.il2 * qa : h2;
+                if constexpr (CORE == 1) {
+                    const float w = s.core + h2;
+                    const float G = side ? w : w * qa;
+                    ubar = (s.g * num) * (F * __builtin_amdgcn_rcpf(fmaxf(G, FLT_MIN)));
+                } else if constexpr (CORE == 2) {
+                    const float w = fmaxf(fmaf(h2, h2, s.core), FLT_MIN);
+                    const float G = side ? w : w * (qa * qa);
+                    ubar = (s.g * num) * (F * __builtin_amdgcn_rsqf(G));
+                } else {
+                    float hn = h2;
+                    for (int i = 1; i < vat_n; ++i) hn = hn * h2;
+                    const float w = fmaxf(s.core + hn, FLT_MIN);
+                    const float root = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(w) * neg_inv_n);
+                    const float G = side ? 1.0f : qa;
+                    ubar = (s.g * num) * (F * root * __builtin_amdgcn_rcpf(G));
+                }
             }
             bool zero = near;
             if constexpr (SEM == 1) zero = zero || (s1 > co2 && s2 > co2);

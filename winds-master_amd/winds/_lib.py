@@ -54,6 +54,15 @@ def lib():
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} not found: the HIP engine is not built "
                            "(run __graft_entry__.build()); there is no CPU fallback")
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7).
+    # Loading torch first makes the engine bind to that copy, so the process
+    # holds one HIP runtime and torch's streams and allocations are valid in
+    # engine calls; loading the engine first would hand torch a runtime it was
+    # not built against.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(str(LIB_PATH))
     L.winds_bs_opts_default.argtypes = [C.POINTER(BsOpts)]
     L.winds_bs_opts_default.restype = None
