@@ -81,8 +81,35 @@ def test_fp32c_config1(c1, cmod, d):
                               co=1000.0)
         s = stats(u, ref)
         flush_print(f"   {s}")
-        # FP32 pair arithmetic, compensated sums, FP64 near pairs
-        assert s["max_rel"] <= 2e-5 and s["rms_rel"] <= 1e-6 and s["normwise"] <= 1e-6
+        # FP32 pair arithmetic, compensated sums, FP64 near pairs.  A shed-only
+        # or trail-only call leaves many targets with |u| near 1e-3 of the
+        # largest, where the relative error is the absolute error magnified;
+        # the 1e-6 per-target bound is asserted on whole evaluations below.
+        assert s["max_rel"] <= 5e-6 and s["rms_rel"] <= 3e-7 and s["normwise"] <= 1e-7
+
+
+@pytest.mark.parametrize("core", ["vatistas2", "vatistas1", "cutoff"])
+def test_fp32c_config1_whole_wake(c1, core):
+    import torch
+    from winds.device import Evaluator
+    recs = layout.wake_segments(c1)
+    tg = torch.from_numpy(layout.wake_targets(c1)).cuda().contiguous()
+    sg = torch.from_numpy(recs).cuda().contiguous()
+    F1 = np.asfortranarray(recs[:, 0:3].reshape(-1, 3, 1, 1))
+    F2 = np.asfortranarray(recs[:, 3:6].reshape(-1, 3, 1, 1))
+    if core == "cutoff":
+        o = winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, d=1e-4,
+                       precision=winds.PREC_FP32C)
+        ref = O.bs_mex(F1, F2, c1.domain, recs[:, 6], 1e-4)
+    else:
+        n = int(core[-1])
+        o = winds.opts(core=winds.CORE_VATISTAS, vat_n=n, semantics=winds.SEM_MATLAB,
+                       precision=winds.PREC_FP32C)
+        ref = O.bs_matlab(F1, F2, c1.domain, recs[:, 6], recs[:, 7], f"visc{n}")
+    u = layout.uind_to_cm(Evaluator(0).eval(tg, sg, o).cpu().numpy(), c1.domain.shape)
+    s = stats(u, ref)
+    flush_print(f"[fp32c whole wake {core}] {s}")
+    assert s["max_rel"] <= 1e-6 and s["rms_rel"] <= 1e-7 and s["normwise"] <= 1e-7
 
 
 # --------------------------------------------------------------------------
@@ -199,3 +226,45 @@ def test_device_api_matches_host_api(c1):
         assert s["normwise"] <= 1e-6
     for x in outs[1:]:
         assert np.max(np.abs(x - outs[0])) <= 1e-7 * np.max(np.abs(outs[0]))
+
+
+# --------------------------------------------------------------------------
+# full single-wake size: 133,920 targets x 263,427 segments
+# --------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def c2_device(need_gpu):
+    import torch
+    w = gen.config2()
+    recs = layout.wake_segments(w)
+    tg = torch.from_numpy(layout.wake_targets(w)).cuda().contiguous()
+    sg = torch.from_numpy(recs).cuda().contiguous()
+    return w, recs, tg, sg
+
+
+@pytest.mark.parametrize("core", ["vatistas2", "cutoff"])
+def test_config2_fp32c_within_1e6_of_fp64(c2_device, core):
+    from winds.device import Evaluator
+    w, recs, tg, sg = c2_device
+    ev = Evaluator(0)
+    if core == "cutoff":
+        kw = dict(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, d=1e-4)
+    else:
+        kw = dict(core=winds.CORE_VATISTAS, vat_n=2, semantics=winds.SEM_MATLAB)
+    flush_print(f"[config2 {core}] fp64")
+    u64 = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP64, **kw)).cpu().numpy()
+    flush_print(f"[config2 {core}] fp32c")
+    u32 = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP32C, **kw)).cpu().numpy()
+    s = stats(u32, u64)
+    flush_print(f"   {s}")
+    assert s["max_rel"] <= 1e-6 and s["rms_rel"] <= 1e-7 and s["normwise"] <= 1e-6
+    # the FP64 engine itself against the CPU oracle on a sample of targets
+    idx = np.random.default_rng(5).choice(w.n_targets, size=24, replace=False)
+    P = np.asfortranarray(layout.wake_targets(w)[:, idx].T.reshape(-1, 3, 1, 1))
+    F1 = np.asfortranarray(recs[:, 0:3].reshape(-1, 3, 1, 1))
+    F2 = np.asfortranarray(recs[:, 3:6].reshape(-1, 3, 1, 1))
+    if core == "cutoff":
+        ref = O.bs_mex(F1, F2, P, recs[:, 6], 1e-4)
+    else:
+        ref = O.bs_matlab(F1, F2, P, recs[:, 6], recs[:, 7], "visc2")
+    assert np.array_equal(u64[:, idx].T, ref[:, :, 0, 0])

@@ -74,8 +74,12 @@ __global__ __launch_bounds__(1024) void bs_fp64_matlab(
             for (int64_t k = c0; k < c1; ++k) {
                 const Seg64 s = seg[k];
                 double ex, ey, ez;
-                matlab_pair(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, s.rc, px, py, pz, d,
-                            n, co, ex, ey, ez);
+                if (n != 0)
+                    matlab_pair<true>(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, s.rc, px, py, pz,
+                                      d, n, co, ex, ey, ez);
+                else
+                    matlab_pair<false>(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, s.rc, px, py, pz,
+                                       d, 0, co, ex, ey, ez);
                 cx += ex;
                 cy += ey;
                 cz += ez;
@@ -110,7 +114,7 @@ __global__ void prep_seg32(const Seg64 *__restrict__ seg, int64_t nseg,
         r.ax = r.ay = r.az = 1.0e6;
         r.bx = r.by = r.bz = 1.0e6;
         r.lx = r.ly = r.lz = 0.0f;
-        r.l2 = 0.0f;
+        r.near2b = -1.0f;
         r.il2 = 0.0f;
         r.g = 0.0f;
         r.core = 1.0f;
@@ -130,7 +134,6 @@ __global__ void prep_seg32(const Seg64 *__restrict__ seg, int64_t nseg,
     r.lx = (float)lx;
     r.ly = (float)ly;
     r.lz = (float)lz;
-    r.l2 = (float)l2;
     r.il2 = l2 > 0.0 ? (float)(1.0 / l2) : 0.0f;
     r.g = l2 > 0.0 ? (float)(s.g / (4.0 * kMlPi)) : 0.0f;
     double core;
@@ -145,6 +148,7 @@ __global__ void prep_seg32(const Seg64 *__restrict__ seg, int64_t nseg,
     if (nl2 > nr2) nr2 = nl2;
     if (!(nr2 > 0.0) || !(s.g == s.g)) nr2 = __builtin_inf();
     r.near2 = (float)nr2;
+    r.near2b = (float)(nr2 + l2);
     out[k] = r;
 }
 
@@ -158,8 +162,9 @@ __device__ __forceinline__ void two_sum_acc(float &hi, float &lo, float v) {
 
 // CORE: 0 cut-off, 1 Vatistas n=1, 2 Vatistas n=2, 3 Vatistas general n.
 // SEM:  0 MEX semantics (near pairs by biot_p2p), 1 BiotSavart.m semantics
-//       (co cut-off; near pairs by the BiotSavart.m arithmetic with zeroing).
-template <int CORE, int SEM>
+//       (near pairs by the BiotSavart.m arithmetic with zeroing).
+// CO:   apply the far cut-off r1 > co && r2 > co (BiotSavart.m:111).
+template <int CORE, int SEM, bool CO>
 __global__ __launch_bounds__(256) void bs_fp32c(
     const double *__restrict__ tx, const double *__restrict__ ty,
     const double *__restrict__ tz, int64_t nt, const Seg32 *__restrict__ s32,
@@ -181,8 +186,12 @@ __global__ __launch_bounds__(256) void bs_fp32c(
             const Seg32 s = s32[k];
             const float r1x = (float)(px - s.ax), r1y = (float)(py - s.ay),
                         r1z = (float)(pz - s.az);
+#if WINDS_R2_FROM_L
+            const float r2x = r1x - s.lx, r2y = r1y - s.ly, r2z = r1z - s.lz;
+#else
             const float r2x = (float)(px - s.bx), r2y = (float)(py - s.by),
                         r2z = (float)(pz - s.bz);
+#endif
             const float s1 = fmaf(r1z, r1z, fmaf(r1y, r1y, r1x * r1x));
             const float s2 = fmaf(r2z, r2z, fmaf(r2y, r2y, r2x * r2x));
             const float dot = fmaf(r1z, r2z, fmaf(r1y, r2y, r1x * r2x));
@@ -191,11 +200,11 @@ __global__ __launch_bounds__(256) void bs_fp32c(
             const float cy = fmaf(s.lz, r1x, -(s.lx * r1z));
             const float cz = fmaf(s.lx, r1y, -(s.ly * r1x));
             const float c2 = fmaf(cz, cz, fmaf(cy, cy, cx * cx));  // |r1 x r2|^2
-            const float tp = s1 - dot;  // r1 . L
-            // squared distance from the target to the closed segment
-            const float tc = __builtin_amdgcn_fmed3f(tp, 0.0f, s.l2);
-            const float d2 = fmaf(-(tc * fmaf(2.0f, tp, -tc)), s.il2, s1);
-            const bool near = !(d2 >= s.near2);
+            const float h2 = c2 * s.il2;  // squared distance to the segment's line
+            // Conservative near test: every target closer than R to the
+            // closed segment has h2 < R^2 and min(s1, s2) < R^2 + |L|^2.
+            // NaN anywhere selects the FP64 branch.
+            const bool near = !(h2 >= s.near2) && !(fminf(s1, s2) >= s.near2b);
             const float i1 = __builtin_amdgcn_rsqf(s1);
             const float i2 = __builtin_amdgcn_rsqf(s2);
             const float pp = i1 * i2;
@@ -214,7 +223,6 @@ __global__ __launch_bounds__(256) void bs_fp32c(
                 const float deno = side ? fmaf(t1, qa, c2) : (qa + t1);
                 ubar = ((s.g * num) * nume) * __builtin_amdgcn_rcpf(deno);
             } else {
-                const float h2 = c2 * s.il2;  // squared distance to the line, >= 0
                 // C/den with C the Vatistas factor: side: il2 qa C/h2,
                 // otherwise C/qa
                 const float F = side ? s.il2 * qa : h2;
@@ -236,7 +244,7 @@ __global__ __launch_bounds__(256) void bs_fp32c(
                 }
             }
             bool zero = near;
-            if constexpr (SEM == 1) zero = zero || (s1 > co2 && s2 > co2);
+            if constexpr (CO) zero = zero || (s1 > co2 && s2 > co2);
             ubar = zero ? 0.0f : ubar;
             qx = fmaf(ubar, cx, qx);
             qy = fmaf(ubar, cy, qy);
@@ -247,8 +255,8 @@ __global__ __launch_bounds__(256) void bs_fp32c(
                     mex_pair(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, px, py, pz, d, ax, ay, az);
                 } else {
                     double ex, ey, ez;
-                    matlab_pair(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, q.rc, px, py, pz, d,
-                                CORE == 0 ? 0 : vat_n, co, ex, ey, ez);
+                    matlab_pair<CORE != 0, (CORE == 1 || CORE == 2) ? CORE : 0>(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, q.rc, px, py,
+                                           pz, d, vat_n, co, ex, ey, ez);
                     ax += ex;
                     ay += ey;
                     az += ez;
@@ -383,13 +391,13 @@ hipError_t launch_fp64(const double *t3, int64_t nt, const Seg64 *seg,
 
 int64_t seg32_padded(int64_t nseg) { return (nseg + kSegTile - 1) / kSegTile * kSegTile; }
 
-template <int CORE, int SEM>
+template <int CORE, int SEM, bool CO>
 static void fp32c_dispatch(dim3 grid, int block, hipStream_t st, const double *tx,
                            const double *ty, const double *tz, int64_t nt,
                            const Seg32 *s32, const Seg64 *s64, int64_t npad,
                            int64_t chunk, float co2, double d, int n, double co,
                            float ninv, double *out, int64_t stride) {
-    bs_fp32c<CORE, SEM><<<grid, block, 0, st>>>(tx, ty, tz, nt, s32, s64, npad, chunk, co2, d, n,
+    bs_fp32c<CORE, SEM, CO><<<grid, block, 0, st>>>(tx, ty, tz, nt, s32, s64, npad, chunk, co2, d, n,
                                                 co, ninv, out, stride);
 }
 
@@ -417,17 +425,25 @@ hipError_t launch_fp32c(const double *t3, int64_t nt, const Seg64 *seg,
     const float co2 = (c.semantics == 1 && cof < 1.0e18) ? (float)(cof * cof) : __builtin_inff();
     const float ninv = n > 0 ? -1.0f / (float)n : 0.0f;
     const int kind = c.core == 0 ? 0 : (n == 1 ? 1 : (n == 2 ? 2 : 3));
-#define WINDS_FP32C(K, S)                                                                   \
-    fp32c_dispatch<K, S>(grid, block, st, tx, ty, tz, nt, s32, seg, npad, chunk, co2, c.d, n, \
-                         c.co, ninv, out, nt)
+#define WINDS_FP32C(K, S, C)                                                                \
+    fp32c_dispatch<K, S, C>(grid, block, st, tx, ty, tz, nt, s32, seg, npad, chunk, co2, c.d, n, \
+                            c.co, ninv, out, nt)
+    const bool has_co = c.semantics == 1 && c.co < 1.0e18;
     if (c.semantics == 0) {
-        WINDS_FP32C(0, 0);
+        WINDS_FP32C(0, 0, false);
+    } else if (has_co) {
+        switch (kind) {
+            case 0: WINDS_FP32C(0, 1, true); break;
+            case 1: WINDS_FP32C(1, 1, true); break;
+            case 2: WINDS_FP32C(2, 1, true); break;
+            default: WINDS_FP32C(3, 1, true); break;
+        }
     } else {
         switch (kind) {
-            case 0: WINDS_FP32C(0, 1); break;
-            case 1: WINDS_FP32C(1, 1); break;
-            case 2: WINDS_FP32C(2, 1); break;
-            default: WINDS_FP32C(3, 1); break;
+            case 0: WINDS_FP32C(0, 1, false); break;
+            case 1: WINDS_FP32C(1, 1, false); break;
+            case 2: WINDS_FP32C(2, 1, false); break;
+            default: WINDS_FP32C(3, 1, false); break;
         }
     }
 #undef WINDS_FP32C

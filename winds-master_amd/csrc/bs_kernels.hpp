@@ -10,7 +10,15 @@ namespace winds {
 
 // Segments per compensated partial sum in the FP32 kernel; segment tables
 // are padded to a multiple of it.
-constexpr int kSegTile = 32;
+#ifndef WINDS_SEG_TILE
+#define WINDS_SEG_TILE 16
+#endif
+constexpr int kSegTile = WINDS_SEG_TILE;
+
+// r2 = P - B formed in FP64 and rounded (0), or as r1 - L in FP32 (1).
+#ifndef WINDS_R2_FROM_L
+#define WINDS_R2_FROM_L 0
+#endif
 
 // Prepared segment record of the FP32 kernel (80 bytes, read with scalar
 // loads).  The endpoints stay FP64 so that r1 = P-A and r2 = P-B are formed
@@ -19,7 +27,7 @@ struct __attribute__((aligned(16))) Seg32 {
     double ax, ay, az;   // F1 endpoint
     double bx, by, bz;   // F2 endpoint
     float lx, ly, lz;    // L = B - A, rounded once
-    float l2;            // |L|^2
+    float near2b;        // near2 + |L|^2 (second half of the near test)
     float il2;           // 1/|L|^2 (0 for a zero-length segment)
     float g;             // gamma / (4 pi) (0 for a zero-length segment)
     float core;          // cut-off: d |L|^2; Vatistas: rc^n (rc = r_c^2)
@@ -35,8 +43,8 @@ struct EvalCfg {
     int precision = 0;   // 0 FP64, 1 FP32C
     double d = 1e-4;
     double co = __builtin_inf();
-    double near_factor = 5.0;
-    double near_len_frac = 1.0 / 16.0;
+    double near_factor = 15.0;
+    double near_len_frac = 0.25;
     int block = 0;
     int split = 0;
     int64_t seg_col = 0;

@@ -62,10 +62,29 @@ __device__ __forceinline__ double inv_root(double y, int n) {
     return pow(y, -1.0 / (double)n);
 }
 
+// The same with the order fixed at compile time (NF = 1 or 2) or taken at
+// run time (NF = 0).
+template <int NF>
+__device__ __forceinline__ double inv_root_n(double y, int n) {
+    if constexpr (NF == 1) return 1.0 / y;
+    if constexpr (NF == 2) return 1.0 / sqrt(y);
+    return inv_root(y, n);
+}
+
+template <int NF>
+__device__ __forceinline__ double ipow_n(double x, int n) {
+    if constexpr (NF == 1) return x;
+    if constexpr (NF == 2) return x * x;
+    return ipow(x, n);
+}
+
 // BiotSavart.m:86-115 for one pair: cut-off core (n == 0) or Vatistas
 // order n (n >= 1, rc = r_c^2), each expression evaluated left to right as
 // MATLAB does, ubar zeroed when NaN/Inf or when r1 > co and r2 > co
-// (BiotSavart.m:111).  Returns the term ubar * (r1 x r2).
+// (BiotSavart.m:111).  Returns the term ubar * (r1 x r2).  VISC selects the
+// Vatistas branch at compile time (n >= 1), otherwise the cut-off (n = 0);
+// NF = 1 or 2 fixes the Vatistas order at compile time, NF = 0 reads n.
+template <bool VISC, int NF = 0>
 __device__ __forceinline__ void matlab_pair(double x1, double y1, double z1,
                                             double x2, double y2, double z2,
                                             double g, double rc, double px,
@@ -81,11 +100,11 @@ __device__ __forceinline__ void matlab_pair(double x1, double y1, double z1,
     double r1dr2 = pxx1 * pxx2 + pyy1 * pyy2 + pzz1 * pzz2;
     double r1tr2 = r1 * r2;
     double ubar;
-    if (n != 0) {
+    if (VISC) {
         double ld = x2x1 * pxx1 + y2y1 * pyy1 + z2z1 * pzz1;
         double Ldr12 = ld * ld;
         double Cnu = (r1 * r1) - (Ldr12 / L);
-        Cnu = Cnu * inv_root(ipow(rc, n) + ipow(Cnu, n), n);
+        Cnu = Cnu * inv_root_n<NF>(ipow_n<NF>(rc, n) + ipow_n<NF>(Cnu, n), n);
         ubar = Cnu * g / (4.0 * kMlPi) * (r1 + r2) / (r1tr2 * (r1tr2 + r1dr2));
     } else {
         ubar = (g / (4.0 * kMlPi)) * (r1 + r2) / (r1tr2 * (r1tr2 + r1dr2) + (d * L));
