@@ -147,6 +147,49 @@ int winds_segment_length_dev(const double *segments, int64_t n_segments,
 /* Number of visible HIP devices (0 when no GPU); never fails. */
 int winds_device_count(void);
 
+/*
+ * Batched evaluation of many independent wakes of uneven size in one
+ * persistent launch.  No counterpart exists in the reference, which evaluates
+ * one wake per BiotSavartMex call (BiotSavartMex.cu:452-685); this is the
+ * same operator applied to wake w's own targets and segments for every w.
+ *
+ * Wake w owns targets [t_off, t_off + n_targets[w]) of one concatenated SoA
+ * target array (t_off = sum of earlier n_targets) and segment records
+ * [s_off, s_off + n_segments[w]) of one concatenated record array.  uind is
+ * SoA over all targets.  The plan holds the work list (wake, target tile,
+ * segment chunk) in longest-first order; a persistent kernel pulls items
+ * from an atomic counter; per-item partial sums are reduced per target in a
+ * fixed order, so results do not depend on scheduling.  The precision must
+ * be WINDS_PREC_FP32C.
+ */
+typedef struct winds_batch winds_batch;
+
+typedef struct winds_batch_tune {
+    int32_t block;      /* targets per work item = threads per workgroup (64, 128, 256) */
+    int32_t chunk;      /* segments per work item (multiple of 16)                      */
+    int32_t wg_per_cu;  /* persistent workgroups per compute unit; 0 = by occupancy     */
+    int32_t reserved;
+} winds_batch_tune;
+
+/* tune == NULL selects block 256, chunk 16384, workgroups by occupancy. */
+int winds_batch_create(int32_t n_wakes, const int64_t *n_targets,
+                       const int64_t *n_segments, const winds_bs_opts *o,
+                       const winds_batch_tune *tune, winds_batch **out);
+void winds_batch_destroy(winds_batch *b);
+size_t winds_batch_workspace_bytes(const winds_batch *b);
+/* Enqueue one evaluation on stream.  use_graph != 0 records the evaluation
+ * into a hipGraph on first use (per set of pointers) and replays it after. */
+int winds_batch_eval_dev(winds_batch *b, const double *targets, const double *segments,
+                         double *uind, void *workspace, size_t workspace_bytes,
+                         void *stream, int use_graph);
+/* Time each candidate launch shape on these inputs, keep the fastest in b,
+ * and report it (and its time in ms) through best / best_ms. */
+int winds_batch_autotune(winds_batch *b, const double *targets, const double *segments,
+                         double *uind, void *stream, winds_batch_tune *best,
+                         double *best_ms);
+/* Totals: targets, segments, interactions (sum over wakes of nt*ns), items. */
+int winds_batch_info(const winds_batch *b, int64_t out[4]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,6 +14,7 @@
 #include <mutex>
 #include <string>
 
+#include "bs_common.hpp"
 #include "bs_kernels.hpp"
 #include "winds_bs.h"
 
@@ -33,6 +34,16 @@ int fail(int code, const std::string &msg) {
     g_err = msg;
     return code;
 }
+
+}  // namespace
+
+namespace winds {
+int set_error(int code, const std::string &msg) { return fail(code, msg); }
+void clear_error() { g_err.clear(); }
+int device_cu_count();
+}  // namespace winds
+
+namespace {
 
 #define WINDS_HIP(expr)                                                             \
     do {                                                                            \
@@ -87,7 +98,11 @@ int check_device(int dev) {
     return WINDS_OK;
 }
 
-int cu_count() {
+int cu_count() { return winds::device_cu_count(); }
+
+}  // namespace
+
+int winds::device_cu_count() {
     static int cached = 0;
     if (cached) return cached;
     int dev = 0, n = 0;
@@ -98,6 +113,8 @@ int cu_count() {
         cached = 256;
     return cached;
 }
+
+namespace {
 
 bool dims_ok(const int64_t *d) {
     for (int i = 0; i < 4; ++i)
@@ -412,3 +429,5 @@ int winds_bs_eval(const double *F1, const double *F2, const int64_t fdims[4], co
 }
 
 }  // extern "C"
+
+int winds::opts_to_cfg(const winds_bs_opts *o, winds::EvalCfg &c) { return to_cfg(o, c); }

@@ -1,0 +1,20 @@
+// bs_common.hpp -- error reporting shared by the C-ABI translation units.
+#pragma once
+
+#include <string>
+
+namespace winds {
+// Record the message of a failing call on this thread and return code.
+int set_error(int code, const std::string &msg);
+void clear_error();
+// Compute units of the current device (cached; 256 if unknown).
+int device_cu_count();
+}  // namespace winds
+
+#include "bs_kernels.hpp"
+#include "winds_bs.h"
+
+namespace winds {
+// Validate public options and convert them to the kernel configuration.
+int opts_to_cfg(const winds_bs_opts *o, EvalCfg &c);
+}  // namespace winds

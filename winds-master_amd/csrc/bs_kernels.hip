@@ -28,6 +28,7 @@
 #include <float.h>
 #include <stdint.h>
 
+#include "bs_fp32c.hpp"
 #include "bs_kernels.hpp"
 #include "bs_pair.hpp"
 
@@ -107,170 +108,27 @@ __global__ void prep_seg32(const Seg64 *__restrict__ seg, int64_t nseg,
                            Seg32 *__restrict__ out) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nseg_pad) return;
-    Seg32 r;
-    if (k >= nseg) {
-        // Padding record: far away, zero strength, never near.  Every FP32
-        // intermediate stays finite, so its term is exactly zero.
-        r.ax = r.ay = r.az = 1.0e6;
-        r.bx = r.by = r.bz = 1.0e6;
-        r.lx = r.ly = r.lz = 0.0f;
-        r.near2b = -1.0f;
-        r.il2 = 0.0f;
-        r.g = 0.0f;
-        r.core = 1.0f;
-        r.near2 = -1.0f;
-        out[k] = r;
-        return;
-    }
-    const Seg64 s = seg[k];
-    const double lx = s.bx - s.ax, ly = s.by - s.ay, lz = s.bz - s.az;
-    const double l2 = lx * lx + ly * ly + lz * lz;
-    r.ax = s.ax;
-    r.ay = s.ay;
-    r.az = s.az;
-    r.bx = s.bx;
-    r.by = s.by;
-    r.bz = s.bz;
-    r.lx = (float)lx;
-    r.ly = (float)ly;
-    r.lz = (float)lz;
-    r.il2 = l2 > 0.0 ? (float)(1.0 / l2) : 0.0f;
-    r.g = l2 > 0.0 ? (float)(s.g / (4.0 * kMlPi)) : 0.0f;
-    double core;
-    if (core_kind == 0) {
-        core = d * l2;
-    } else {
-        core = ipow(s.rc, n);
-    }
-    r.core = (float)core;
-    double nr2 = near_factor * near_factor * s.rc;
-    const double nl2 = near_len_frac * near_len_frac * l2;
-    if (nl2 > nr2) nr2 = nl2;
-    if (!(nr2 > 0.0) || !(s.g == s.g)) nr2 = __builtin_inf();
-    r.near2 = (float)nr2;
-    r.near2b = (float)(nr2 + l2);
-    out[k] = r;
+    out[k] = make_seg32(k < nseg ? seg + k : nullptr, core_kind, n, d, near_factor, near_len_frac);
 }
 
-__device__ __forceinline__ void two_sum_acc(float &hi, float &lo, float v) {
-    const float s = hi + v;
-    const float bb = s - hi;
-    const float err = (hi - (s - bb)) + (v - bb);
-    hi = s;
-    lo += err;
-}
-
-// CORE: 0 cut-off, 1 Vatistas n=1, 2 Vatistas n=2, 3 Vatistas general n.
-// SEM:  0 MEX semantics (near pairs by biot_p2p), 1 BiotSavart.m semantics
-//       (near pairs by the BiotSavart.m arithmetic with zeroing).
-// CO:   apply the far cut-off r1 > co && r2 > co (BiotSavart.m:111).
+// One launch over a single segment set: grid.x covers targets, grid.y
+// splits the segment range; split partials go to out[split][3][stride].
 template <int CORE, int SEM, bool CO>
 __global__ __launch_bounds__(256) void bs_fp32c(
     const double *__restrict__ tx, const double *__restrict__ ty,
     const double *__restrict__ tz, int64_t nt, const Seg32 *__restrict__ s32,
-    const Seg64 *__restrict__ s64, int64_t nseg_pad, int64_t chunk, float co2,
-    double d, int vat_n, double co, float neg_inv_n, double *__restrict__ out,
-    int64_t out_stride) {
+    const Seg64 *__restrict__ s64, int64_t nseg_pad, int64_t chunk, Fp32Params P,
+    double *__restrict__ out, int64_t out_stride) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     const int64_t k0 = (int64_t)blockIdx.y * chunk;
     const int64_t k1 = k0 + chunk < nseg_pad ? k0 + chunk : nseg_pad;
-    const double px = tx[t], py = ty[t], pz = tz[t];
-    float hx = 0.f, hy = 0.f, hz = 0.f, lx = 0.f, ly = 0.f, lz = 0.f;
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int64_t kt = k0; kt < k1; kt += kSegTile) {
-        float qx = 0.f, qy = 0.f, qz = 0.f;
-#pragma unroll 2
-        for (int j = 0; j < kSegTile; ++j) {
-            const int64_t k = kt + j;
-            const Seg32 s = s32[k];
-            const float r1x = (float)(px - s.ax), r1y = (float)(py - s.ay),
-                        r1z = (float)(pz - s.az);
-#if WINDS_R2_FROM_L
-            const float r2x = r1x - s.lx, r2y = r1y - s.ly, r2z = r1z - s.lz;
-#else
-            const float r2x = (float)(px - s.bx), r2y = (float)(py - s.by),
-                        r2z = (float)(pz - s.bz);
-#endif
-            const float s1 = fmaf(r1z, r1z, fmaf(r1y, r1y, r1x * r1x));
-            const float s2 = fmaf(r2z, r2z, fmaf(r2y, r2y, r2x * r2x));
-            const float dot = fmaf(r1z, r2z, fmaf(r1y, r2y, r1x * r2x));
-            // r1 x r2 = L x r1 with L = B - A.
-            const float cx = fmaf(s.ly, r1z, -(s.lz * r1y));
-            const float cy = fmaf(s.lz, r1x, -(s.lx * r1z));
-            const float cz = fmaf(s.lx, r1y, -(s.ly * r1x));
-            const float c2 = fmaf(cz, cz, fmaf(cy, cy, cx * cx));  // |r1 x r2|^2
-            const float h2 = c2 * s.il2;  // squared distance to the segment's line
-            // Conservative near test: every target closer than R to the
-            // closed segment has h2 < R^2 and min(s1, s2) < R^2 + |L|^2.
-            // NaN anywhere selects the FP64 branch.
-            const bool near = !(h2 >= s.near2) && !(fminf(s1, s2) >= s.near2b);
-            const float i1 = __builtin_amdgcn_rsqf(s1);
-            const float i2 = __builtin_amdgcn_rsqf(s2);
-            const float pp = i1 * i2;
-            const float r1r2 = (s1 * s2) * pp;
-            // r1 r2 + r1.r2 cancels when the target sits beside the segment
-            // (r1.r2 < 0); there it equals |r1 x r2|^2 / (r1 r2 - r1.r2),
-            // whose terms are both accurate.  qa = r1 r2 + |r1.r2| never cancels.
-            const float qa = r1r2 + fabsf(dot);
-            const bool side = dot <= 0.0f;
-            const float num = i1 + i2;
-            float ubar;
-            if constexpr (CORE == 0) {
-                // 1/(r1r2 + dot + d|L|^2/(r1r2)), in the form matching side
-                const float t1 = s.core * pp;
-                const float nume = side ? qa : 1.0f;
-                const float deno = side ? fmaf(t1, qa, c2) : (qa + t1);
-                ubar = ((s.g * num) * nume) * __builtin_amdgcn_rcpf(deno);
-            } else {
-                // C/den with C the Vatistas factor: side: il2 qa C/h2,
-                // otherwise C/qa
-                const float F = side ? s.il2 * qa : h2;
-                if constexpr (CORE == 1) {
-                    const float w = s.core + h2;
-                    const float G = side ? w : w * qa;
-                    ubar = (s.g * num) * (F * __builtin_amdgcn_rcpf(fmaxf(G, FLT_MIN)));
-                } else if constexpr (CORE == 2) {
-                    const float w = fmaxf(fmaf(h2, h2, s.core), FLT_MIN);
-                    const float G = side ? w : w * (qa * qa);
-                    ubar = (s.g * num) * (F * __builtin_amdgcn_rsqf(G));
-                } else {
-                    float hn = h2;
-                    for (int i = 1; i < vat_n; ++i) hn = hn * h2;
-                    const float w = fmaxf(s.core + hn, FLT_MIN);
-                    const float root = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(w) * neg_inv_n);
-                    const float G = side ? 1.0f : qa;
-                    ubar = (s.g * num) * (F * root * __builtin_amdgcn_rcpf(G));
-                }
-            }
-            bool zero = near;
-            if constexpr (CO) zero = zero || (s1 > co2 && s2 > co2);
-            ubar = zero ? 0.0f : ubar;
-            qx = fmaf(ubar, cx, qx);
-            qy = fmaf(ubar, cy, qy);
-            qz = fmaf(ubar, cz, qz);
-            if (near) {
-                const Seg64 q = s64[k];
-                if constexpr (SEM == 0) {
-                    mex_pair(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, px, py, pz, d, ax, ay, az);
-                } else {
-                    double ex, ey, ez;
-                    matlab_pair<CORE != 0, (CORE == 1 || CORE == 2) ? CORE : 0>(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, q.rc, px, py,
-                                           pz, d, vat_n, co, ex, ey, ez);
-                    ax += ex;
-                    ay += ey;
-                    az += ez;
-                }
-            }
-        }
-        two_sum_acc(hx, lx, qx);
-        two_sum_acc(hy, ly, qy);
-        two_sum_acc(hz, lz, qz);
-    }
+    double sx, sy, sz;
+    fp32c_range<CORE, SEM, CO>(tx[t], ty[t], tz[t], s32, s64, 0, k0, k1, P, sx, sy, sz);
     double *o = out + (int64_t)blockIdx.y * 3 * out_stride;
-    o[t] = ((double)hx + (double)lx) + ax;
-    o[out_stride + t] = ((double)hy + (double)ly) + ay;
-    o[2 * out_stride + t] = ((double)hz + (double)lz) + az;
+    o[t] = sx;
+    o[out_stride + t] = sy;
+    o[2 * out_stride + t] = sz;
 }
 
 __global__ void bs_reduce_split(const double *__restrict__ part, int64_t nt,
@@ -395,10 +253,9 @@ template <int CORE, int SEM, bool CO>
 static void fp32c_dispatch(dim3 grid, int block, hipStream_t st, const double *tx,
                            const double *ty, const double *tz, int64_t nt,
                            const Seg32 *s32, const Seg64 *s64, int64_t npad,
-                           int64_t chunk, float co2, double d, int n, double co,
-                           float ninv, double *out, int64_t stride) {
-    bs_fp32c<CORE, SEM, CO><<<grid, block, 0, st>>>(tx, ty, tz, nt, s32, s64, npad, chunk, co2, d, n,
-                                                co, ninv, out, stride);
+                           int64_t chunk, const Fp32Params &P, double *out, int64_t stride) {
+    bs_fp32c<CORE, SEM, CO><<<grid, block, 0, st>>>(tx, ty, tz, nt, s32, s64, npad, chunk, P, out,
+                                                    stride);
 }
 
 hipError_t launch_fp32c(const double *t3, int64_t nt, const Seg64 *seg,
@@ -425,9 +282,14 @@ hipError_t launch_fp32c(const double *t3, int64_t nt, const Seg64 *seg,
     const float co2 = (c.semantics == 1 && cof < 1.0e18) ? (float)(cof * cof) : __builtin_inff();
     const float ninv = n > 0 ? -1.0f / (float)n : 0.0f;
     const int kind = c.core == 0 ? 0 : (n == 1 ? 1 : (n == 2 ? 2 : 3));
+    Fp32Params P;
+    P.co2 = co2;
+    P.neg_inv_n = ninv;
+    P.vat_n = n;
+    P.d = c.d;
+    P.co = c.co;
 #define WINDS_FP32C(K, S, C)                                                                \
-    fp32c_dispatch<K, S, C>(grid, block, st, tx, ty, tz, nt, s32, seg, npad, chunk, co2, c.d, n, \
-                            c.co, ninv, out, nt)
+    fp32c_dispatch<K, S, C>(grid, block, st, tx, ty, tz, nt, s32, seg, npad, chunk, P, out, nt)
     const bool has_co = c.semantics == 1 && c.co < 1.0e18;
     if (c.semantics == 0) {
         WINDS_FP32C(0, 0, false);

@@ -19,7 +19,9 @@ EXPORTS = (
     "winds_bs_opts_default", "winds_last_error", "winds_version", "winds_bs_eval",
     "winds_bs_workspace_bytes", "winds_bs_eval_dev", "winds_pack_segments_dev",
     "winds_pack_targets_dev", "winds_unpack_uind_dev", "winds_segment_length_dev",
-    "winds_device_count",
+    "winds_device_count", "winds_batch_create", "winds_batch_destroy",
+    "winds_batch_workspace_bytes", "winds_batch_eval_dev", "winds_batch_autotune",
+    "winds_batch_info",
 )
 
 

@@ -22,6 +22,7 @@ LIB = LIBDIR / "libwinds_bs.so"
 SOURCES = [
     "bs_kernels.hip",
     "bs_api.cpp",
+    "bs_batch.hip",
 ]
 
 ARCH = "gfx950"
