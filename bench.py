@@ -255,7 +255,7 @@ def main():
         "config": {
             "workload": (f"single large wake, FP32 compensated vs FP64: 3 blades x 31 stations x "
                          f"{args.ages} ages, {nt} targets x {nseg} segments (shed+trail) per "
-                         f"evaluation, Vatistas n=2 core, FP64 for pairs within max(15 r_c, |L|/4)"),
+                         f"evaluation, Vatistas n=2 core, FP64 for pairs within max(40 r_c, |L|/2)"),
             "targets": nt, "segments": nseg, "interactions_per_step": inter,
             "core": "vatistas2", "precision": "fp32 compensated + fp64 near pairs",
         },

@@ -62,7 +62,7 @@ typedef struct winds_bs_opts {
     double co;             /* far cut-off distance (SEM_MATLAB); +inf = none */
     double near_factor;    /* FP32C: FP64 for pairs with the target within   */
     double near_len_frac;  /*   max(near_factor*r_c, near_len_frac*|L|) of   */
-                           /*   the segment; <= 0 selects 15 and 1/4         */
+                           /*   the segment; <= 0 selects 40 and 1/2         */
     int32_t block;         /* threads per block; 0 = tuned                   */
     int32_t split;         /* segment-range split (FP32C); 0 = tuned         */
     int32_t seg_col;       /* SEM_MATLAB+FP64: segments per station column   */

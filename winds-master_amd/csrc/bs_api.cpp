@@ -23,12 +23,13 @@ namespace {
 thread_local std::string g_err;
 
 // Default near radius of the FP32 kernel: pairs whose target lies within
-// max(15 r_c, |L|/4) of the segment are evaluated in FP64.  Chosen by
-// measurement on the 133,920-target single-wake workload: per-target
+// max(40 r_c, |L|/2) of the segment are evaluated in FP64.  Chosen by
+// measurement on the 133,920-target single-wake workload and on smaller
+// wakes of other TSR, yaw and size: per-target
 // relative error below 1e-6 against FP64 for the cut-off and Vatistas cores,
-// at under 1% of the evaluation time.
-constexpr double kNearFactor = 15.0;
-constexpr double kNearLenFrac = 0.25;
+// for about 5% of the evaluation time.
+constexpr double kNearFactor = 40.0;
+constexpr double kNearLenFrac = 0.5;
 
 int fail(int code, const std::string &msg) {
     g_err = msg;

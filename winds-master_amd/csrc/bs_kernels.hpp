@@ -43,8 +43,8 @@ struct EvalCfg {
     int precision = 0;   // 0 FP64, 1 FP32C
     double d = 1e-4;
     double co = __builtin_inf();
-    double near_factor = 15.0;
-    double near_len_frac = 0.25;
+    double near_factor = 40.0;
+    double near_len_frac = 0.5;
     int block = 0;
     int split = 0;
     int64_t seg_col = 0;
