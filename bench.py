@@ -144,6 +144,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-wake batch")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one plain 2 GiB device-to-device copy (reference point for "
+                         "the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--ages", type=int, default=1440)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -235,6 +238,13 @@ def main():
     torch.cuda.synchronize()
     ms_c = c0.elapsed_time(c1) / nc
     rate_c = inter / (ms_c * 1e-3) / 1e9
+
+    if args.calib_copy:
+        src = torch.ones(2 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     batch = None if args.no_batch else batch_bench(o32, o64, args.steps)
 
