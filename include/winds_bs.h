@@ -64,7 +64,9 @@ typedef struct winds_bs_opts {
     double near_len_frac;  /*   max(near_factor*r_c, near_len_frac*|L|) of   */
                            /*   the segment; <= 0 selects 40 and 1/2         */
     int32_t block;         /* threads per block; 0 = tuned                   */
-    int32_t split;         /* segment-range split (FP32C); 0 = tuned         */
+    int32_t split;         /* segment-range split; FP32C: 0 = tuned; FP64:  */
+                           /*   0/1 = one sequential sum (reference order),  */
+                           /*   > 1 = split sums (for few targets)           */
     int32_t seg_col;       /* SEM_MATLAB+FP64: segments per station column   */
     int32_t seg_blk;       /*   and per blade, for the nested MATLAB sum     */
 } winds_bs_opts;

@@ -16,8 +16,8 @@ from winds import _lib
 ROOT = Path(__file__).resolve().parents[1]
 
 
-def header_functions():
-    text = (ROOT / "include" / "winds_bs.h").read_text()
+def header_functions(name="winds_bs.h"):
+    text = (ROOT / "include" / name).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(winds_[a-z0-9_]+)\s*\(", text)))
 
@@ -29,6 +29,10 @@ def test_every_declared_symbol_is_exported():
     for n in names:
         assert hasattr(L, n), n
     assert set(_lib.EXPORTS) == set(names)
+    march = header_functions("winds_march.h")
+    assert len(march) >= 10
+    for n in march:
+        assert hasattr(L, n), n
 
 
 def test_version_and_defaults():

@@ -184,8 +184,10 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 size_t ws_bytes(int64_t nt, int64_t nseg, const winds::EvalCfg &c, int *split_out) {
     if (c.precision != WINDS_PREC_FP32C) {
-        if (split_out) *split_out = 1;
-        return 256;
+        // FP64 splits only when asked to (the sequential sum is the reference order)
+        const int split = c.split > 1 ? c.split : 1;
+        if (split_out) *split_out = split;
+        return split > 1 ? align256((size_t)split * 3 * (size_t)nt * sizeof(double)) + 256 : 256;
     }
     const int split = c.split > 0 ? c.split : pick_split(nt, nseg, c.block);
     if (split_out) *split_out = split;
@@ -199,7 +201,14 @@ int eval_dev_impl(const double *t3, int64_t nt, const winds::Seg64 *seg, int64_t
     int split = 1;
     const size_t need = ws_bytes(nt, nseg, c, &split);
     if (c.precision == WINDS_PREC_FP64) {
-        WINDS_HIP(winds::launch_fp64(t3, nt, seg, nseg, c, u3, st));
+        double *part = nullptr;
+        if (split > 1) {
+            if (!ws || wsb < need)
+                return fail(WINDS_E_NOMEM, "workspace of " + std::to_string(wsb) + " bytes; need " +
+                                               std::to_string(need));
+            part = (double *)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+        }
+        WINDS_HIP(winds::launch_fp64(t3, nt, seg, nseg, c, u3, st, part));
         return WINDS_OK;
     }
     if (!ws || wsb < need)
@@ -432,3 +441,12 @@ int winds_bs_eval(const double *F1, const double *F2, const int64_t fdims[4], co
 }  // extern "C"
 
 int winds::opts_to_cfg(const winds_bs_opts *o, winds::EvalCfg &c) { return to_cfg(o, c); }
+
+int winds::eval_dev(const double *t3, int64_t nt, const winds::Seg64 *seg, int64_t nseg,
+                    const winds::EvalCfg &c, double *u3, void *ws, size_t wsb, hipStream_t st) {
+    return eval_dev_impl(t3, nt, seg, nseg, c, u3, ws, wsb, st);
+}
+
+size_t winds::eval_ws_bytes(int64_t nt, int64_t nseg, const winds::EvalCfg &c) {
+    return ws_bytes(nt, nseg, c, nullptr);
+}

@@ -17,4 +17,9 @@ int device_cu_count();
 namespace winds {
 // Validate public options and convert them to the kernel configuration.
 int opts_to_cfg(const winds_bs_opts *o, EvalCfg &c);
+// Enqueue one evaluation (device layout of winds_bs_eval_dev) on st.
+int eval_dev(const double *t3, int64_t nt, const Seg64 *seg, int64_t nseg, const EvalCfg &c,
+             double *u3, void *ws, size_t wsb, hipStream_t st);
+// Workspace bytes eval_dev needs for these sizes.
+size_t eval_ws_bytes(int64_t nt, int64_t nseg, const EvalCfg &c);
 }  // namespace winds

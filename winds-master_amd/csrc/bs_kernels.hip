@@ -98,6 +98,40 @@ __global__ __launch_bounds__(1024) void bs_fp64_matlab(
     uz[t] = sz;
 }
 
+// FP64 over one segment chunk per grid.y (split-K for few targets); the
+// partial of chunk y goes to part[y][3][nt].  MODE: 0 biot_p2p (MEX),
+// 1 BiotSavart.m cut-off, 2 BiotSavart.m Vatistas order n.  Segments are
+// summed sequentially within a chunk; chunks are added by bs_reduce_split.
+template <int MODE>
+__global__ __launch_bounds__(256) void bs_fp64_split(
+    const double *__restrict__ tx, const double *__restrict__ ty,
+    const double *__restrict__ tz, int64_t nt, const Seg64 *__restrict__ seg,
+    int64_t nseg, int64_t chunk, double d, int n, double co, double *__restrict__ part) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    const int64_t k0 = (int64_t)blockIdx.y * chunk;
+    const int64_t k1 = k0 + chunk < nseg ? k0 + chunk : nseg;
+    const double px = tx[t], py = ty[t], pz = tz[t];
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int64_t k = k0; k < k1; ++k) {
+        const Seg64 s = seg[k];
+        if constexpr (MODE == 0) {
+            mex_pair(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, px, py, pz, d, sx, sy, sz);
+        } else {
+            double ex, ey, ez;
+            matlab_pair<MODE == 2>(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, s.rc, px, py, pz, d,
+                                   n, co, ex, ey, ez);
+            sx += ex;
+            sy += ey;
+            sz += ez;
+        }
+    }
+    double *o = part + (int64_t)blockIdx.y * 3 * nt;
+    o[t] = sx;
+    o[nt + t] = sy;
+    o[2 * nt + t] = sz;
+}
+
 // ---------------------------------------------------------------------------
 // FP32 compensated kernel
 // ---------------------------------------------------------------------------
@@ -229,12 +263,30 @@ hipError_t launch_zero(double *p, int64_t n, hipStream_t st) {
 }
 
 hipError_t launch_fp64(const double *t3, int64_t nt, const Seg64 *seg,
-                       int64_t nseg, const EvalCfg &c, double *u3, hipStream_t st) {
+                       int64_t nseg, const EvalCfg &c, double *u3, hipStream_t st,
+                       double *part) {
     const int block = c.block > 0 ? c.block : 256;
     if (nt <= 0) return hipSuccess;
     if (nseg <= 0) return launch_zero(u3, 3 * nt, st);
     const double *tx = t3, *ty = t3 + nt, *tz = t3 + 2 * nt;
     double *ux = u3, *uy = u3 + nt, *uz = u3 + 2 * nt;
+    if (c.split > 1 && part) {
+        // segment range split over grid.y, partials reduced in split order
+        const int64_t chunk = (nseg + c.split - 1) / c.split;
+        const int nsplit = (int)((nseg + chunk - 1) / chunk);
+        dim3 grid(grid_for(nt, block), nsplit);
+        const int n = c.core == 0 ? 0 : c.vat_n;
+        if (c.semantics == 0)
+            bs_fp64_split<0><<<grid, block, 0, st>>>(tx, ty, tz, nt, seg, nseg, chunk, c.d, 0, c.co, part);
+        else if (n == 0)
+            bs_fp64_split<1><<<grid, block, 0, st>>>(tx, ty, tz, nt, seg, nseg, chunk, c.d, 0, c.co, part);
+        else
+            bs_fp64_split<2><<<grid, block, 0, st>>>(tx, ty, tz, nt, seg, nseg, chunk, c.d, n, c.co, part);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        bs_reduce_split<<<grid_for(nt, 256), 256, 0, st>>>(part, nt, nsplit, ux, uy, uz);
+        return hipGetLastError();
+    }
     if (c.semantics == 0) {
         bs_fp64_mex<<<grid_for(nt, block), block, 0, st>>>(tx, ty, tz, nt, seg, nseg, c.d, ux, uy, uz);
     } else {

@@ -54,8 +54,10 @@ struct EvalCfg {
 int64_t seg32_padded(int64_t nseg);
 
 hipError_t launch_zero(double *p, int64_t n, hipStream_t st);
+// FP64 sum.  split > 1 (with part holding split*3*nt doubles) splits the
+// segment range across workgroups; otherwise one sequential sum per target.
 hipError_t launch_fp64(const double *t3, int64_t nt, const Seg64 *seg, int64_t nseg,
-                       const EvalCfg &c, double *u3, hipStream_t st);
+                       const EvalCfg &c, double *u3, hipStream_t st, double *part = nullptr);
 hipError_t launch_fp32c(const double *t3, int64_t nt, const Seg64 *seg, int64_t nseg,
                         const EvalCfg &c, double *u3, Seg32 *s32, double *part,
                         hipStream_t st);

@@ -1,0 +1,1113 @@
+// march.hip -- device-resident free-vortex-wake time march (include/winds_march.h).
+//
+// State layout for a wake of A ages (MATLAB's column-major order of the
+// reference's cell arrays, flattened):
+//   nodes   SoA x,y,z over j = m + a*nst + b*nst*A      (wake.domain{p})
+//   gs, rs  shed circulation / core^2 over m + a*ns + b*ns*A        [ns,1,A,nb]
+//   gt, rt  trailing over m + a*nst + b*nst*(A-1)               [nst,1,A-1,nb]
+//   len_s/t filament lengths, same layouts (vcore.m:42-46)
+// Segments for the induced velocity are rebuilt every evaluation from nodes
+// and circulation, shed first (F1 = node m, F2 = node m+1), then trailing
+// (F1 = age a+1, F2 = age a), exactly the pairs of WInDS.m:150-154.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "bs_common.hpp"
+#include "bs_kernels.hpp"
+#include "bs_pair.hpp"
+#include "winds_march.h"
+
+namespace winds {
+namespace mk {
+
+constexpr double kAlphaRL = 1.25643;  // core/constants.m:22
+constexpr double kNu = 1e-5;          // :23
+constexpr double kDelta = 100.0;      // :24
+constexpr double kA1 = 6.5e-5;        // :25
+constexpr double kRho = 1.23;         // :28
+constexpr double kPi = 3.141592653589793;
+
+struct Geom {
+    const double *r_trail, *chord_trail, *twist_trail;
+    const double *r_node, *chord_node, *twist_node, *dr_node;
+    const double *alpha, *cl, *cd;
+    int nb, nst, ns, n_alpha;
+    double omega, pitch_deg, ux, uy, uz;
+};
+
+__device__ __forceinline__ double rad(double deg) { return deg * kPi / 180.0; }
+
+// e_r, e_t of blade b at time t (azimuth 2 pi b/nb + omega t)
+__device__ __forceinline__ void frame(const Geom &g, int b, double t, double &cp, double &sp) {
+    const double psi = 2.0 * kPi * b / g.nb + g.omega * t;
+    cp = cos(psi);
+    sp = sin(psi);
+}
+
+// chord direction e_c = -cos(th) e_t + sin(th) e_x, th = twist + pitch
+__device__ __forceinline__ void chord_dir(double cp, double sp, double th_deg, double &cx,
+                                          double &cy, double &cz) {
+    const double th = rad(th_deg);
+    const double ct = cos(th), st = sin(th);
+    cx = st;
+    cy = -ct * -sp;
+    cz = -ct * cp;
+}
+
+// quarter-chord and trailing-edge nodes (ages 0 and 1) of a wake of A ages
+__global__ void k_nodes01(Geom g, double t, int A, double *__restrict__ X) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.nst * g.nb) return;
+    const int m = i % g.nst, b = i / g.nst;
+    const int64_t N = (int64_t)g.nst * A * g.nb;
+    double cp, sp;
+    frame(g, b, t, cp, sp);
+    const double r = g.r_trail[m];
+    const double qx = r * 0.0, qy = r * cp, qz = r * sp;
+    double ex, ey, ez;
+    chord_dir(cp, sp, g.twist_trail[m] + g.pitch_deg, ex, ey, ez);
+    const double c = 0.75 * g.chord_trail[m];
+    const int64_t j0 = m + (int64_t)b * g.nst * A;
+    const int64_t j1 = j0 + g.nst;
+    X[j0] = qx;
+    X[N + j0] = qy;
+    X[2 * N + j0] = qz;
+    X[j1] = qx + c * ex;
+    X[N + j1] = qy + c * ey;
+    X[2 * N + j1] = qz + c * ez;
+}
+
+// lifting-line frame at station m of blade b: bound point, bxn/byn/bzn and
+// the inflow U - Omega r e_t seen in the blade frame (velocity.m:58-70)
+struct LL {
+    double px, py, pz;
+    double bx[3], by[3], bz[3];
+    double vb[3];      // inertial inflow
+    double vblade[3];  // blade-frame inflow
+};
+
+__device__ __forceinline__ LL ll_frame(const Geom &g, int m, int b, double t) {
+    LL L;
+    double cp, sp;
+    frame(g, b, t, cp, sp);
+    const double r = g.r_node[m];
+    L.px = r * 0.0;
+    L.py = r * cp;
+    L.pz = r * sp;
+    chord_dir(cp, sp, g.twist_node[m] + g.pitch_deg, L.bx[0], L.bx[1], L.bx[2]);
+    L.bz[0] = 0.0;
+    L.bz[1] = cp;
+    L.bz[2] = sp;
+    // byn = bzn x bxn
+    L.by[0] = L.bz[1] * L.bx[2] - L.bz[2] * L.bx[1];
+    L.by[1] = L.bz[2] * L.bx[0] - L.bz[0] * L.bx[2];
+    L.by[2] = L.bz[0] * L.bx[1] - L.bz[1] * L.bx[0];
+    const double etx = 0.0, ety = -sp, etz = cp;
+    const double w = g.omega * r;
+    L.vb[0] = g.ux - w * etx;
+    L.vb[1] = g.uy - w * ety;
+    L.vb[2] = g.uz - w * etz;
+    L.vblade[0] = L.bx[0] * L.vb[0] + L.bx[1] * L.vb[1] + L.bx[2] * L.vb[2];
+    L.vblade[1] = L.by[0] * L.vb[0] + L.by[1] * L.vb[1] + L.by[2] * L.vb[2];
+    L.vblade[2] = L.bz[0] * L.vb[0] + L.bz[1] * L.vb[1] + L.bz[2] * L.vb[2];
+    return L;
+}
+
+// interp1(alpha, tab, x) linear, NaN outside the table
+__device__ double interp1(const double *xp, const double *fp, int n, double x) {
+    if (!(x >= xp[0] && x <= xp[n - 1])) return __builtin_nan("");
+    if (x == xp[n - 1]) return fp[n - 1];
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (xp[mid] <= x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const double slope = (fp[lo + 1] - fp[lo]) / (xp[lo + 1] - xp[lo]);
+    return slope * (x - xp[lo]) + fp[lo];
+}
+
+// initials.m:110-125: bound circulation 0.5*|U|*chord*cl(geometric aoa)
+__global__ void k_init_bound(Geom g, double t, double umag, int A, double *__restrict__ gs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.ns * g.nb) return;
+    const int m = i % g.ns, b = i / g.ns;
+    const LL L = ll_frame(g, m, b, t);
+    const double aoa = atan2(-L.vblade[1], L.vblade[0]) * (180.0 / kPi);
+    const double cl0 = interp1(g.alpha, g.cl, g.n_alpha, aoa);
+    gs[m + (int64_t)b * g.ns * A] = 0.5 * umag * g.chord_node[m] * cl0;
+}
+
+// Kelvin (WInDS.m:131-140), trailing part: age 0 = diff([0; bound; 0]),
+// ages 1.. = previous trailing ages 0.. (the oldest dropped when truncated)
+__global__ void k_kelvin_trail(int nst, int nb, const double *__restrict__ gs_old, int A_old,
+                               const double *__restrict__ gt_old, int A_new,
+                               double *__restrict__ gt_new) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ns = nst - 1;
+    const int64_t n = (int64_t)nst * (A_new - 1) * nb;
+    if (i >= n) return;
+    const int m = (int)(i % nst);
+    const int a = (int)((i / nst) % (A_new - 1));
+    const int b = (int)(i / ((int64_t)nst * (A_new - 1)));
+    double v;
+    if (a == 0) {
+        const double *gb = gs_old + (int64_t)b * ns * A_old;  // bound = shed age 0
+        const double hi = m < ns ? gb[m] : 0.0;
+        const double lo = m > 0 ? gb[m - 1] : 0.0;
+        v = hi - lo;
+    } else {
+        v = gt_old[m + (int64_t)(a - 1) * nst + (int64_t)b * nst * (A_old - 1)];
+    }
+    gt_new[i] = v;
+}
+
+// Kelvin, shed part: age 0 = bound; age a>=1 = Cz(a) - C(a-1), C = span
+// cumulative sum of trailing ages, Cz(A-1) = 0 (WInDS.m:139-140)
+__global__ void k_kelvin_shed(int nst, int nb, const double *__restrict__ gs_old, int A_old,
+                              const double *__restrict__ gt_new, int A_new,
+                              double *__restrict__ gs_new) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ns = nst - 1;
+    if (i >= A_new * nb) return;
+    const int a = i % A_new, b = i / A_new;
+    double *out = gs_new + (int64_t)b * ns * A_new + (int64_t)a * ns;
+    if (a == 0) {
+        const double *gb = gs_old + (int64_t)b * ns * A_old;
+        for (int m = 0; m < ns; ++m) out[m] = gb[m];
+        return;
+    }
+    const int nta = A_new - 1;  // trailing ages
+    const double *tp = gt_new + (int64_t)b * nst * nta + (int64_t)(a - 1) * nst;
+    const double *tc = a <= nta - 1 ? tp + nst : nullptr;
+    double cprev = 0.0, ccur = 0.0;
+    for (int m = 0; m < ns; ++m) {
+        cprev += tp[m];
+        if (tc) ccur += tc[m];
+        out[m] = (tc ? ccur : 0.0) - cprev;
+    }
+}
+
+// vcore.m:30-51 + FilamentLength.m + filamentmod.m:27-41 for one time index
+__global__ void k_vcore(int nst, int nb, int A, const double *__restrict__ X,
+                        const double *__restrict__ gs, const double *__restrict__ gt, double r0,
+                        double T, int fmod, const double *__restrict__ ls_prev,
+                        const double *__restrict__ lt_prev, int A_prev,
+                        double *__restrict__ ls, double *__restrict__ lt,
+                        double *__restrict__ rs, double *__restrict__ rt) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ns = nst - 1;
+    const int64_t N = (int64_t)nst * A * nb;
+    const int64_t n_s = (int64_t)ns * A * nb;
+    const int64_t n_t = (int64_t)nst * (A - 1) * nb;
+    if (i >= n_s + n_t) return;
+    const bool shed = i < n_s;
+    const int64_t k = shed ? i : i - n_s;
+    int m, a, b;
+    int64_t j1, j2;  // F1, F2 node indices
+    if (shed) {
+        m = (int)(k % ns);
+        a = (int)((k / ns) % A);
+        b = (int)(k / ((int64_t)ns * A));
+        j1 = m + (int64_t)a * nst + (int64_t)b * nst * A;
+        j2 = j1 + 1;
+    } else {
+        m = (int)(k % nst);
+        a = (int)((k / nst) % (A - 1));
+        b = (int)(k / ((int64_t)nst * (A - 1)));
+        j2 = m + (int64_t)a * nst + (int64_t)b * nst * A;
+        j1 = j2 + nst;
+    }
+    const double dx = X[j2] - X[j1], dy = X[N + j2] - X[N + j1], dz = X[2 * N + j2] - X[2 * N + j1];
+    const double len = sqrt(dx * dx + dy * dy + dz * dz);
+    const double gam = shed ? gs[k] : gt[k];
+    const double Re = fabs(gam / kNu);
+    const double rc = r0 * r0 + 4.0 * kAlphaRL * kNu * (1.0 + kA1 * Re) * T;
+    double rce = rc;
+    if (fmod) {
+        // interior ages 2..end-1 (1-based), compared at the same age index
+        const bool inner = shed ? (a >= 1 && a <= A - 2) : (a >= 1 && a <= A - 3);
+        if (inner) {
+            const int64_t kp = shed ? m + (int64_t)a * ns + (int64_t)b * ns * A_prev
+                                    : m + (int64_t)a * nst + (int64_t)b * nst * (A_prev - 1);
+            const double lold = sqrt(shed ? ls_prev[kp] : lt_prev[kp]);
+            const double lnew = sqrt(len);
+            const double strain = (lnew - lold) / lold;
+            rce = rc * (1.0 / (1.0 + strain));
+        }
+    }
+    if (shed) {
+        ls[k] = len;
+        rs[k] = rce;
+    } else {
+        lt[k] = len;
+        rt[k] = rce;
+    }
+}
+
+// Segment records from nodes and circulation (shed first, then trailing).
+// zero_changing zeroes the circulation of the filaments the Kutta-Joukowski
+// iteration changes: shed ages 0 and 1, trailing age 0.
+__global__ void k_segments(int nst, int nb, int A, const double *__restrict__ X,
+                           const double *__restrict__ gs, const double *__restrict__ gt,
+                           const double *__restrict__ rs, const double *__restrict__ rt,
+                           int zero_changing, Seg64 *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int ns = nst - 1;
+    const int64_t N = (int64_t)nst * A * nb;
+    const int64_t n_s = (int64_t)ns * A * nb;
+    const int64_t n_t = (int64_t)nst * (A - 1) * nb;
+    if (i >= n_s + n_t) return;
+    Seg64 s;
+    int64_t j1, j2;
+    if (i < n_s) {
+        const int m = (int)(i % ns);
+        const int a = (int)((i / ns) % A);
+        const int b = (int)(i / ((int64_t)ns * A));
+        j1 = m + (int64_t)a * nst + (int64_t)b * nst * A;
+        j2 = j1 + 1;
+        s.g = (zero_changing && a <= 1) ? 0.0 : gs[i];
+        s.rc = rs[i];
+    } else {
+        const int64_t k = i - n_s;
+        const int m = (int)(k % nst);
+        const int a = (int)((k / nst) % (A - 1));
+        const int b = (int)(k / ((int64_t)nst * (A - 1)));
+        j2 = m + (int64_t)a * nst + (int64_t)b * nst * A;
+        j1 = j2 + nst;
+        s.g = (zero_changing && a == 0) ? 0.0 : gt[k];
+        s.rc = rt[k];
+    }
+    s.ax = X[j1];
+    s.ay = X[N + j1];
+    s.az = X[2 * N + j1];
+    s.bx = X[j2];
+    s.by = X[N + j2];
+    s.bz = X[2 * N + j2];
+    out[i] = s;
+}
+
+__global__ void k_add_uinf(int64_t n, double ux, double uy, double uz,
+                           const double *__restrict__ u, double *__restrict__ v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    v[i] = ux + u[i];
+    v[n + i] = uy + u[n + i];
+    v[2 * n + i] = uz + u[2 * n + i];
+}
+
+// New ages 2.. from the old nodes: forward Euler (fe.m:31), PCC corrector
+// (pcc.m:74-75, vpred != null) or Adams-Bashforth 4 (ab4.m:38-42, h1 != null)
+__global__ void k_advect(int nst, int nb, int A_old, int A_new, double dt,
+                         const double *__restrict__ Xo, const double *__restrict__ vo,
+                         const double *__restrict__ vpred, const double *__restrict__ h1,
+                         int A1, const double *__restrict__ h2, int A2,
+                         const double *__restrict__ h3, int A3, double *__restrict__ Xn) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int nadv = A_new - 2;
+    if (nadv <= 0) return;
+    const int64_t n = (int64_t)nst * nadv * nb;
+    if (i >= n) return;
+    const int m = (int)(i % nst);
+    const int ai = (int)((i / nst) % nadv);  // slice index, new age ai+2
+    const int b = (int)(i / ((int64_t)nst * nadv));
+    const int64_t No = (int64_t)nst * A_old * nb;
+    const int64_t Nn = (int64_t)nst * A_new * nb;
+    const int64_t jo = m + (int64_t)(ai + 1) * nst + (int64_t)b * nst * A_old;
+    const int64_t jn = m + (int64_t)(ai + 2) * nst + (int64_t)b * nst * A_new;
+    for (int c = 0; c < 3; ++c) {
+        const double src = Xo[c * No + jo];
+        const double v0 = vo[c * No + jo];
+        double x;
+        if (vpred) {
+            x = src + dt / 2 * (vpred[c * Nn + jn] + v0);
+        } else if (h1) {
+            double acc = 55 * v0;
+            const double *hs[3] = {h1, h2, h3};
+            const int As[3] = {A1, A2, A3};
+            const double coef[3] = {-59, 37, -9};
+            for (int k = 1; k <= 3; ++k) {
+                double sh = 0.0;
+                if (ai >= k) {
+                    const int age = 1 + ai - k;
+                    const int64_t Nk = (int64_t)nst * As[k - 1] * nb;
+                    sh = hs[k - 1][c * Nk + m + (int64_t)age * nst + (int64_t)b * nst * As[k - 1]];
+                }
+                acc = acc + coef[k - 1] * sh;
+            }
+            x = src + dt / 24 * acc;
+        } else {
+            x = src + dt * v0;
+        }
+        Xn[c * Nn + jn] = x;
+    }
+}
+
+__global__ void k_ll_points(Geom g, double t, double *__restrict__ P) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = g.ns * g.nb;
+    if (i >= n) return;
+    const LL L = ll_frame(g, i % g.ns, i / g.ns, t);
+    P[i] = L.px;
+    P[n + i] = L.py;
+    P[2 * n + i] = L.pz;
+}
+
+// Unit-circulation influence of the filaments the KJ iteration changes on
+// the lifting-line points: K[(t*NCH + s)*3 + c].  Per blade the changing
+// filaments are shed age 0 (ns), shed age 1 (ns), trailing age 0 (nst).
+__global__ void k_influence(Geom g, int A, const double *__restrict__ X,
+                            const double *__restrict__ rs, const double *__restrict__ rt,
+                            const double *__restrict__ P, int sem, int vat_n, double d, double co,
+                            double *__restrict__ K) {
+    const int nst = g.nst, ns = g.ns, nb = g.nb;
+    const int per = 2 * ns + nst;
+    const int NCH = per * nb;
+    const int NLL = ns * nb;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)NLL * NCH) return;
+    const int t = (int)(i / NCH);
+    const int s = (int)(i % NCH);
+    const int b = s / per, r = s % per;
+    const int64_t N = (int64_t)nst * A * nb;
+    int64_t j1, j2;
+    double rc;
+    if (r < 2 * ns) {
+        const int a = r / ns, m = r % ns;
+        j1 = m + (int64_t)a * nst + (int64_t)b * nst * A;
+        j2 = j1 + 1;
+        rc = rs[m + (int64_t)a * ns + (int64_t)b * ns * A];
+    } else {
+        const int m = r - 2 * ns;
+        j2 = m + (int64_t)b * nst * A;
+        j1 = j2 + nst;
+        rc = rt[m + (int64_t)b * nst * (A - 1)];
+    }
+    const double px = P[t], py = P[NLL + t], pz = P[2 * NLL + t];
+    double ux = 0.0, uy = 0.0, uz = 0.0;
+    if (sem == 0) {
+        mex_pair(X[j1], X[N + j1], X[2 * N + j1], X[j2], X[N + j2], X[2 * N + j2], 1.0, px, py, pz,
+                 d, ux, uy, uz);
+    } else if (vat_n == 0) {
+        matlab_pair<false>(X[j1], X[N + j1], X[2 * N + j1], X[j2], X[N + j2], X[2 * N + j2], 1.0,
+                           rc, px, py, pz, d, 0, co, ux, uy, uz);
+    } else {
+        matlab_pair<true>(X[j1], X[N + j1], X[2 * N + j1], X[j2], X[N + j2], X[2 * N + j2], 1.0,
+                          rc, px, py, pz, d, vat_n, co, ux, uy, uz);
+    }
+    K[i * 3 + 0] = ux;
+    K[i * 3 + 1] = uy;
+    K[i * 3 + 2] = uz;
+}
+
+constexpr int kKjThreads = 256;
+constexpr int kMaxLL = 1024;     // lifting-line points handled by one workgroup
+constexpr int kMaxNodesBlade = 256;
+
+// Kutta-Joukowski fixed point (KuttaJoukowski.m:42-49 with kj() :204-296)
+// for time t, in FP64, in one workgroup, followed by perform.m:42-62.
+// frozen: induced velocity at the lifting line of every filament except the
+// changing ones; K their unit influence.  Writes the bound circulation, the
+// trailing age 0 and shed age 1 circulation, cl/cd/aoa and one history row.
+__global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, double relax,
+                                                  double tol, int maxiter,
+                                                  const double *__restrict__ frozen,
+                                                  const double *__restrict__ K,
+                                                  double *__restrict__ gs, double *__restrict__ gt,
+                                                  double *__restrict__ cl_out,
+                                                  double *__restrict__ cd_out,
+                                                  double *__restrict__ aoa_out,
+                                                  double *__restrict__ hist_row, double p_index,
+                                                  int *__restrict__ flag) {
+    const int nst = g.nst, ns = g.ns, nb = g.nb;
+    const int NLL = ns * nb;
+    const int per = 2 * ns + nst;
+    const int NCH = per * nb;
+    __shared__ double s_g[kMaxLL];      // bound circulation
+    __shared__ double s_gam[3 * kMaxLL + 4 * kMaxNodesBlade];  // changing circulation by s
+    __shared__ double s_c1[kMaxLL];     // span cumsum of trailing age 1
+    __shared__ double s_red[kKjThreads];
+    __shared__ double s_red2[kKjThreads];
+    __shared__ int s_nan;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NLL; i += kKjThreads) {
+        const int m = i % ns, b = i / ns;
+        s_g[i] = gs[m + (int64_t)b * ns * A];
+    }
+    if (tid < nb) {
+        const int b = tid;
+        const double *t1 = gt + (int64_t)b * nst * (A - 1) + nst;  // trailing age 1
+        double c = 0.0;
+        for (int m = 0; m < ns; ++m) {
+            if (A - 1 >= 2) c += t1[m];
+            s_c1[b * ns + m] = A - 1 >= 2 ? c : 0.0;
+        }
+    }
+    if (tid == 0) s_nan = 0;
+    __syncthreads();
+    double dgmax = 1.0;
+    int j = 0;
+    while (dgmax > tol && j < maxiter) {
+        // circulation of the changing filaments from the current bound values
+        if (tid < nb) {
+            const int b = tid;
+            double *sg = s_gam + b * per;
+            double c0 = 0.0;
+            for (int m = 0; m < nst; ++m) {
+                const double hi = m < ns ? s_g[b * ns + m] : 0.0;
+                const double lo = m > 0 ? s_g[b * ns + m - 1] : 0.0;
+                const double tr0 = hi - lo;
+                sg[2 * ns + m] = tr0;
+                if (m < ns) {
+                    c0 += tr0;
+                    sg[m] = s_g[b * ns + m];
+                    sg[ns + m] = s_c1[b * ns + m] - c0;
+                }
+            }
+        }
+        __syncthreads();
+        double gnew_i[kMaxLL / kKjThreads];
+        double local_max = 0.0;
+        int q = 0;
+        for (int i = tid; i < NLL; i += kKjThreads, ++q) {
+            const int m = i % ns, b = i / ns;
+            double ux = frozen[i], uy = frozen[NLL + i], uz = frozen[2 * NLL + i];
+            const double *Ki = K + (int64_t)i * NCH * 3;
+            for (int s = 0; s < NCH; ++s) {
+                const double gm = s_gam[s];
+                ux += gm * Ki[3 * s];
+                uy += gm * Ki[3 * s + 1];
+                uz += gm * Ki[3 * s + 2];
+            }
+            const LL L = ll_frame(g, m, b, t);
+            const double rx = L.bx[0] * ux + L.bx[1] * uy + L.bx[2] * uz;
+            const double ry = L.by[0] * ux + L.by[1] * uy + L.by[2] * uz;
+            const double tu = L.vblade[0] + rx, tv = L.vblade[1] + ry;
+            const double Vinf = sqrt(L.vblade[0] * L.vblade[0] + L.vblade[1] * L.vblade[1] +
+                                     L.vblade[2] * L.vblade[2]);
+            const double aoa = atan2(-tv, tu) * (180.0 / kPi);
+            const double cl = interp1(g.alpha, g.cl, g.n_alpha, aoa);
+            const double cd = interp1(g.alpha, g.cd, g.n_alpha, aoa);
+            if (cl != cl) s_nan = 1;
+            const double gn = 0.5 * Vinf * g.chord_node[m] * cl;
+            const double dg = gn - s_g[i];
+            gnew_i[q] = s_g[i] + relax * dg;
+            const double dgn = fabs(dg / (fabs(gn) + 1.0));
+            local_max = fmax(local_max, dgn);
+            cl_out[i] = cl;  // the last iteration's values are the step's result
+            cd_out[i] = cd;
+            aoa_out[i] = aoa;
+        }
+        s_red[tid] = local_max;
+        __syncthreads();
+        q = 0;
+        for (int i = tid; i < NLL; i += kKjThreads, ++q) s_g[i] = gnew_i[q];
+        for (int w = kKjThreads / 2; w > 0; w >>= 1) {
+            if (tid < w) s_red[tid] = fmax(s_red[tid], s_red[tid + w]);
+            __syncthreads();
+        }
+        dgmax = s_red[0];
+        if (s_nan) dgmax = 0.0;  // diverging: stop, the host reports it
+        ++j;
+        __syncthreads();
+    }
+    // write back the bound, trailing age 0 and shed age 1 circulation
+    if (tid < nb) {
+        const int b = tid;
+        double c0 = 0.0;
+        double *gtb = gt + (int64_t)b * nst * (A - 1);
+        double *gsb = gs + (int64_t)b * ns * A;
+        for (int m = 0; m < nst; ++m) {
+            const double hi = m < ns ? s_g[b * ns + m] : 0.0;
+            const double lo = m > 0 ? s_g[b * ns + m - 1] : 0.0;
+            const double tr0 = hi - lo;
+            gtb[m] = tr0;
+            if (m < ns) {
+                c0 += tr0;
+                gsb[m] = s_g[b * ns + m];
+                if (A >= 2) gsb[ns + m] = (A - 1 >= 2 ? s_c1[b * ns + m] : 0.0) - c0;
+            }
+        }
+    }
+    __syncthreads();
+    // perform.m:42-62 from this time's cl, cd, aoa
+    double dT = 0.0, dQ = 0.0;
+    for (int i = tid; i < NLL; i += kKjThreads) {
+        const int m = i % ns, b = i / ns;
+        double cp, sp;
+        frame(g, b, t, cp, sp);
+        const double r = g.r_node[m];
+        const double etx = 0.0, ety = -sp, etz = cp;
+        const double vbx = g.ux - g.omega * r * etx;
+        const double vby = g.uy - g.omega * r * ety;
+        const double vbz = g.uz - g.omega * r * etz;
+        const double v1 = vbx;
+        const double v2 = -(vbx * etx + vby * ety + vbz * etz);
+        const double tsr = v2 / v1;
+        const double phi = aoa_out[i] - (g.pitch_deg + g.twist_node[m]);
+        const double tphi = tan(rad(phi));
+        const double aap = tsr / tphi;
+        const double a = aap * (v1 - tphi * v2) / (aap * v1 + tphi * v2);
+        const double ap = a / aap;
+        const double cph = cos(rad(phi)), sph = sin(rad(phi));
+        const double dCZ = cl_out[i] * cph + cd_out[i] * sph;
+        const double dCX = cl_out[i] * sph - cd_out[i] * cph;
+        const double w1 = v1 * (1 - a), w2 = v2 * (1 + ap);
+        const double W2 = w1 * w1 + w2 * w2;
+        const double c = g.chord_node[m];
+        dT += 0.5 * kRho * c * dCZ * W2 * g.dr_node[m];
+        dQ += 0.5 * kRho * c * dCX * W2 * r * g.dr_node[m];
+    }
+    s_red[tid] = dT;
+    s_red2[tid] = dQ;
+    __syncthreads();
+    for (int w = kKjThreads / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            s_red[tid] += s_red[tid + w];
+            s_red2[tid] += s_red2[tid + w];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        hist_row[0] = p_index;
+        hist_row[1] = s_red[0];
+        hist_row[2] = s_red2[0];
+        hist_row[3] = g.omega * s_red2[0];
+        hist_row[4] = (double)j;
+        if (s_nan) *flag = 1;
+    }
+}
+
+}  // namespace mk
+}  // namespace winds
+
+using namespace winds::mk;
+
+namespace {
+
+struct DBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t b) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, b < 256 ? 256 : b);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    double *d() const { return (double *)p; }
+};
+
+unsigned blocks(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+struct winds_march {
+    winds_march_cfg cfg{};
+    winds::EvalCfg wake, ll;
+    Geom g{};
+    hipStream_t st = nullptr;
+    int p = 1;        // time index of nodes[cur] (the next step to run)
+    int A = 2;        // ages of nodes[cur]
+    int cur = 0;      // nodes[cur]: time p
+    int gprev = 0;    // gs/gt[gprev]: circulation of time p-1 after its KJ solve
+    int A_gprev = 1;  // ages of gs/gt[gprev]
+    int lcur = 0;     // ls/lt[lcur]: filament lengths of time p-1
+    int nsteps_done = 0;
+    double r0 = 0.0;
+    DBuf geom, nodes[2], vel, velp, uind, hist[3];
+    int hist_A[3] = {0, 0, 0};
+    int n_hist = 0;
+    DBuf gs[2], gt[2], gs1, gt1, rs, rt, rs1, rt1, ls[3], lt[3];
+    int ls_A[3] = {0, 0, 0};
+    DBuf segs, ws, llp, frozen, K, kjout, perf, flag;
+};
+
+namespace {
+
+using winds::set_error;
+
+#define WM_HIP(expr)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return set_error(WINDS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int64_t nodes_of(const winds_march *m, int A) { return (int64_t)m->g.nst * A * m->g.nb; }
+int64_t segs_of(const winds_march *m, int A) {
+    return (int64_t)m->g.ns * A * m->g.nb + (int64_t)m->g.nst * (A - 1) * m->g.nb;
+}
+
+int ensure_ws(winds_march *m, int64_t nt, int64_t nseg, const winds::EvalCfg &c) {
+    const size_t need = winds::eval_ws_bytes(nt, nseg, c);
+    if (need > m->ws.bytes) {
+        WM_HIP(hipStreamSynchronize(m->st));
+        WM_HIP(m->ws.alloc(need + (need >> 3)));
+    }
+    return WINDS_OK;
+}
+
+// Induced velocity at targets t3 (nt) of the wake (nodes, circulation, cores)
+int induced(winds_march *m, const double *X, int A, const double *gsp, const double *gtp,
+            const double *rsp, const double *rtp, const double *t3, int64_t nt, double *u3,
+            const winds::EvalCfg &c, int zero_changing) {
+    const int64_t ns = segs_of(m, A);
+    k_segments<<<blocks(ns), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, X, gsp, gtp, rsp, rtp,
+                                              zero_changing, (winds::Seg64 *)m->segs.p);
+    WM_HIP(hipGetLastError());
+    int rc = ensure_ws(m, nt, ns, c);
+    if (rc) return rc;
+    return winds::eval_dev(t3, nt, (const winds::Seg64 *)m->segs.p, ns, c, u3, m->ws.p,
+                           m->ws.bytes, m->st);
+}
+
+int kelvin(winds_march *m, const double *gs_old, int A_old, const double *gt_old, int A_new,
+           double *gs_new, double *gt_new) {
+    const int64_t nt = (int64_t)m->g.nst * (A_new - 1) * m->g.nb;
+    k_kelvin_trail<<<blocks(nt), 256, 0, m->st>>>(m->g.nst, m->g.nb, gs_old, A_old, gt_old, A_new,
+                                                  gt_new);
+    WM_HIP(hipGetLastError());
+    k_kelvin_shed<<<blocks((int64_t)A_new * m->g.nb), 256, 0, m->st>>>(m->g.nst, m->g.nb, gs_old,
+                                                                         A_old, gt_new, A_new,
+                                                                         gs_new);
+    WM_HIP(hipGetLastError());
+    return WINDS_OK;
+}
+
+int vcore(winds_march *m, int p, const double *X, int A, const double *gsp, const double *gtp,
+          int lprev, int lout, double *rsp, double *rtp) {
+    const double T = (p - 1) * m->cfg.dt;
+    const int fmod = p > 3 ? 1 : 0;
+    const int64_t n = segs_of(m, A);
+    k_vcore<<<blocks(n), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, X, gsp, gtp, m->r0, T, fmod,
+                                          m->ls[lprev].d(), m->lt[lprev].d(),
+                                          std::max(2, m->ls_A[lprev]), m->ls[lout].d(),
+                                          m->lt[lout].d(), rsp, rtp);
+    WM_HIP(hipGetLastError());
+    m->ls_A[lout] = A;
+    return WINDS_OK;
+}
+
+int advect(winds_march *m, const double *Xo, int A_old, const double *vo, const double *vpred,
+           bool ab4, double t_next, double *Xn, int A_new) {
+    k_nodes01<<<blocks((int64_t)m->g.nst * m->g.nb), 256, 0, m->st>>>(m->g, t_next, A_new, Xn);
+    WM_HIP(hipGetLastError());
+    const int64_t n = (int64_t)m->g.nst * std::max(0, A_new - 2) * m->g.nb;
+    if (n > 0) {
+        const double *h1 = nullptr, *h2 = nullptr, *h3 = nullptr;
+        int a1 = 0, a2 = 0, a3 = 0;
+        if (ab4) {
+            // the last pushed history entry is time p-1
+            auto slot = [&](int k) { return (m->n_hist - k) % 3; };
+            h1 = m->hist[slot(1)].d();
+            a1 = m->hist_A[slot(1)];
+            h2 = m->hist[slot(2)].d();
+            a2 = m->hist_A[slot(2)];
+            h3 = m->hist[slot(3)].d();
+            a3 = m->hist_A[slot(3)];
+        }
+        k_advect<<<blocks(n), 256, 0, m->st>>>(m->g.nst, m->g.nb, A_old, A_new, m->cfg.dt, Xo, vo,
+                                               vpred, h1, a1, h2, a2, h3, a3, Xn);
+        WM_HIP(hipGetLastError());
+    }
+    return WINDS_OK;
+}
+
+int push_hist(winds_march *m, const double *vel, int A) {
+    const int slot = m->n_hist % 3;
+    WM_HIP(hipMemcpyAsync(m->hist[slot].p, vel, (size_t)3 * nodes_of(m, A) * sizeof(double),
+                          hipMemcpyDeviceToDevice, m->st));
+    m->hist_A[slot] = A;
+    m->n_hist++;
+    return WINDS_OK;
+}
+
+// initials.m:44-152 (time index 1)
+int setup(winds_march *m) {
+    const int A = 2;
+    double *X = m->nodes[0].d();
+    k_nodes01<<<blocks((int64_t)m->g.nst * m->g.nb), 256, 0, m->st>>>(m->g, 0.0, A, X);
+    WM_HIP(hipGetLastError());
+    const double umag = sqrt(m->cfg.uinf[0] * m->cfg.uinf[0] + m->cfg.uinf[1] * m->cfg.uinf[1] +
+                             m->cfg.uinf[2] * m->cfg.uinf[2]);
+    // the bound circulation as a one-age shed array feeds the Kelvin step
+    k_init_bound<<<blocks((int64_t)m->g.ns * m->g.nb), 256, 0, m->st>>>(m->g, 0.0, umag, 1,
+                                                                        m->gs1.d());
+    WM_HIP(hipGetLastError());
+    int rc = kelvin(m, m->gs1.d(), 1, m->gt1.d(), A, m->gs[0].d(), m->gt[0].d());
+    if (rc) return rc;
+    rc = vcore(m, 1, X, A, m->gs[0].d(), m->gt[0].d(), 1, 0, m->rs.d(), m->rt.d());
+    if (rc) return rc;
+    const int64_t N = nodes_of(m, A);
+    rc = induced(m, X, A, m->gs[0].d(), m->gt[0].d(), m->rs.d(), m->rt.d(), X, N, m->uind.d(),
+                 m->wake, 0);
+    if (rc) return rc;
+    k_add_uinf<<<blocks(N), 256, 0, m->st>>>(N, m->cfg.uinf[0], m->cfg.uinf[1], m->cfg.uinf[2],
+                                             m->uind.d(), m->vel.d());
+    WM_HIP(hipGetLastError());
+    rc = push_hist(m, m->vel.d(), A);
+    if (rc) return rc;
+    const int A_new = std::min(A + 1, m->cfg.max_ages);
+    rc = advect(m, X, A, m->vel.d(), nullptr, false, m->cfg.dt, m->nodes[1].d(), A_new);
+    if (rc) return rc;
+    m->cur = 1;
+    m->A = A_new;
+    m->gprev = 0;
+    m->A_gprev = A;
+    m->lcur = 0;
+    m->p = 2;
+    return WINDS_OK;
+}
+
+// WInDS.m:127-183, one time index p
+int step_once(winds_march *m) {
+    const int p = m->p;
+    const double t = (p - 1) * m->cfg.dt;
+    const int A = m->A;
+    const int nc = m->cur, nn = 1 - m->cur;        // nodes: time p / time p+1
+    const int gp = m->gprev, gc = 1 - m->gprev;    // circulation: time p-1 / time p
+    double *X = m->nodes[nc].d();
+    double *gs = m->gs[gc].d(), *gt = m->gt[gc].d();
+    int rc = kelvin(m, m->gs[gp].d(), m->A_gprev, m->gt[gp].d(), A, gs, gt);
+    if (rc) return rc;
+    const int lprev = m->lcur, lnow = (m->lcur + 1) % 3, lpred = (m->lcur + 2) % 3;
+    rc = vcore(m, p, X, A, gs, gt, lprev, lnow, m->rs.d(), m->rt.d());
+    if (rc) return rc;
+    const int64_t N = nodes_of(m, A);
+    rc = induced(m, X, A, gs, gt, m->rs.d(), m->rt.d(), X, N, m->uind.d(), m->wake, 0);
+    if (rc) return rc;
+    k_add_uinf<<<blocks(N), 256, 0, m->st>>>(N, m->cfg.uinf[0], m->cfg.uinf[1], m->cfg.uinf[2],
+                                             m->uind.d(), m->vel.d());
+    WM_HIP(hipGetLastError());
+    const int A_new = std::min(A + 1, m->cfg.max_ages);
+    double *Xn = m->nodes[nn].d();
+    const int integ = m->cfg.integrator;
+    if (integ == WINDS_INTEG_AB4 && p >= 4) {
+        rc = advect(m, X, A, m->vel.d(), nullptr, true, t + m->cfg.dt, Xn, A_new);
+    } else {
+        rc = advect(m, X, A, m->vel.d(), nullptr, false, t + m->cfg.dt, Xn, A_new);
+        if (!rc && integ == WINDS_INTEG_PCC && p > 2) {
+            // pcc.m:40-75: Kelvin and vcore for p+1 at the predicted nodes,
+            // induced velocity there, trapezoidal corrector
+            rc = kelvin(m, gs, A, gt, A_new, m->gs1.d(), m->gt1.d());
+            if (!rc)
+                rc = vcore(m, p + 1, Xn, A_new, m->gs1.d(), m->gt1.d(), lnow, lpred, m->rs1.d(),
+                           m->rt1.d());
+            const int64_t Nn = nodes_of(m, A_new);
+            if (!rc)
+                rc = induced(m, Xn, A_new, m->gs1.d(), m->gt1.d(), m->rs1.d(), m->rt1.d(), Xn, Nn,
+                             m->uind.d(), m->wake, 0);
+            if (!rc) {
+                k_add_uinf<<<blocks(Nn), 256, 0, m->st>>>(Nn, m->cfg.uinf[0], m->cfg.uinf[1],
+                                                          m->cfg.uinf[2], m->uind.d(), m->velp.d());
+                WM_HIP(hipGetLastError());
+                rc = advect(m, X, A, m->vel.d(), m->velp.d(), false, t + m->cfg.dt, Xn, A_new);
+            }
+        }
+    }
+    if (rc) return rc;
+    // Kutta-Joukowski at time p on the wake of time p (KuttaJoukowski.m:215-219)
+    const int NLL = m->g.ns * m->g.nb;
+    k_ll_points<<<blocks(NLL), 256, 0, m->st>>>(m->g, t, m->llp.d());
+    WM_HIP(hipGetLastError());
+    rc = induced(m, X, A, gs, gt, m->rs.d(), m->rt.d(), m->llp.d(), NLL, m->frozen.d(), m->ll, 1);
+    if (rc) return rc;
+    const int NCH = (2 * m->g.ns + m->g.nst) * m->g.nb;
+    k_influence<<<blocks((int64_t)NLL * NCH), 256, 0, m->st>>>(
+        m->g, A, X, m->rs.d(), m->rt.d(), m->llp.d(), m->ll.semantics,
+        m->ll.core == 0 ? 0 : m->ll.vat_n, m->ll.d, m->ll.co, m->K.d());
+    WM_HIP(hipGetLastError());
+    const int row = m->nsteps_done;
+    if (row >= m->cfg.max_steps) return set_error(WINDS_E_NOMEM, "performance history is full");
+    double *kj = m->kjout.d();
+    k_kj<<<1, kKjThreads, 0, m->st>>>(m->g, t, A, m->cfg.relax, m->cfg.tol, m->cfg.maxiter,
+                                       m->frozen.d(), m->K.d(), gs, gt, kj, kj + NLL, kj + 2 * NLL,
+                                       m->perf.d() + 5 * (int64_t)row, (double)p, (int *)m->flag.p);
+    WM_HIP(hipGetLastError());
+    rc = push_hist(m, m->vel.d(), A);
+    if (rc) return rc;
+    m->nsteps_done++;
+    m->lcur = lnow;
+    m->gprev = gc;
+    m->A_gprev = A;
+    m->cur = nn;
+    m->A = A_new;
+    m->p = p + 1;
+    return WINDS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void winds_march_destroy(winds_march *m) {
+    if (!m) return;
+    if (m->st) (void)hipStreamSynchronize(m->st);
+    if (m->st) (void)hipStreamDestroy(m->st);
+    delete m;
+}
+
+int winds_march_create(const winds_march_cfg *cfg, winds_march **out) {
+    winds::clear_error();
+    if (!cfg || !out) return set_error(WINDS_E_ARG, "null argument");
+    *out = nullptr;
+    const winds_march_cfg &c = *cfg;
+    if (c.nb < 1 || c.nst < 2 || c.max_ages < 3 || c.n_alpha < 2 || c.max_steps < 1)
+        return set_error(WINDS_E_DIMS, "bad march sizes");
+    if ((int64_t)(c.nst - 1) * c.nb > kMaxLL || c.nst > kMaxNodesBlade)
+        return set_error(WINDS_E_DIMS, "too many lifting-line points for the KJ solver");
+    if (!c.r_trail || !c.chord_trail || !c.twist_trail || !c.alpha || !c.cl || !c.cd)
+        return set_error(WINDS_E_ARG, "null geometry or polar");
+    if (c.integrator < WINDS_INTEG_PCC || c.integrator > WINDS_INTEG_AB4)
+        return set_error(WINDS_E_MODE, "unknown integrator");
+    if (!(c.dt > 0.0) || !(c.tsr > 0.0) || !(c.R > 0.0)) return set_error(WINDS_E_ARG, "bad dt/tsr/R");
+    winds_march *m = new winds_march();
+    m->cfg = c;
+    int rc = winds::opts_to_cfg(&c.wake, m->wake);
+    if (!rc) rc = winds::opts_to_cfg(&c.ll, m->ll);
+    if (rc) {
+        delete m;
+        return rc;
+    }
+    // lifting-line sums have few targets: FP64 splits the segment range
+    if (m->ll.precision == WINDS_PREC_FP64 && m->ll.split <= 1) m->ll.split = 256;
+    const int nst = c.nst, ns = c.nst - 1, nb = c.nb, na = c.n_alpha;
+    // geometry on the device: r,chord,twist (nst each), node r,chord,twist,dr (ns each), polar
+    std::vector<double> h(3 * nst + 4 * ns + 3 * na);
+    double *q = h.data();
+    std::copy(c.r_trail, c.r_trail + nst, q);
+    std::copy(c.chord_trail, c.chord_trail + nst, q + nst);
+    std::copy(c.twist_trail, c.twist_trail + nst, q + 2 * nst);
+    double *rn = q + 3 * nst, *cn = rn + ns, *tn = cn + ns, *dr = tn + ns;
+    for (int i = 0; i < ns; ++i) {
+        // NRELrotor.m:48-51: RNodes = midpoints, linear interpolation, |diff|
+        rn[i] = c.r_trail[i] + (c.r_trail[i + 1] - c.r_trail[i]) / 2;
+        const double w = (rn[i] - c.r_trail[i]) / (c.r_trail[i + 1] - c.r_trail[i]);
+        cn[i] = c.chord_trail[i] + w * (c.chord_trail[i + 1] - c.chord_trail[i]);
+        tn[i] = c.twist_trail[i] + w * (c.twist_trail[i + 1] - c.twist_trail[i]);
+        dr[i] = fabs(c.r_trail[i + 1] - c.r_trail[i]);
+    }
+    double *al = dr + ns;
+    std::copy(c.alpha, c.alpha + na, al);
+    std::copy(c.cl, c.cl + na, al + na);
+    std::copy(c.cd, c.cd + na, al + 2 * na);
+    auto fail = [&](int code) {
+        winds_march_destroy(m);
+        return code;
+    };
+#define WMC(expr)                                                                           \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(set_error(WINDS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_))); \
+    } while (0)
+    WMC(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
+    WMC(m->geom.alloc(h.size() * sizeof(double)));
+    WMC(hipMemcpy(m->geom.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    const double *G = m->geom.d();
+    m->g.r_trail = G;
+    m->g.chord_trail = G + nst;
+    m->g.twist_trail = G + 2 * nst;
+    m->g.r_node = G + 3 * nst;
+    m->g.chord_node = m->g.r_node + ns;
+    m->g.twist_node = m->g.chord_node + ns;
+    m->g.dr_node = m->g.twist_node + ns;
+    m->g.alpha = m->g.dr_node + ns;
+    m->g.cl = m->g.alpha + na;
+    m->g.cd = m->g.cl + na;
+    m->g.nb = nb;
+    m->g.nst = nst;
+    m->g.ns = ns;
+    m->g.n_alpha = na;
+    m->g.omega = c.omega;
+    m->g.pitch_deg = c.pitch_deg;
+    m->g.ux = c.uinf[0];
+    m->g.uy = c.uinf[1];
+    m->g.uz = c.uinf[2];
+    const double umag = sqrt(c.uinf[0] * c.uinf[0] + c.uinf[1] * c.uinf[1] + c.uinf[2] * c.uinf[2]);
+    // initials.m:134-135
+    const double T0 = 2 * kPi * c.R / (12 * c.tsr * umag);
+    m->r0 = sqrt(4 * kAlphaRL * kNu * kDelta * T0);
+    const int AM = c.max_ages;
+    const size_t NB = (size_t)3 * nst * AM * nb * sizeof(double);
+    const size_t SS = (size_t)ns * AM * nb * sizeof(double);
+    const size_t ST = (size_t)nst * AM * nb * sizeof(double);
+    const size_t SEG = ((size_t)ns * AM * nb + (size_t)nst * AM * nb) * sizeof(winds::Seg64);
+    const int NLL = ns * nb;
+    const int NCH = (2 * ns + nst) * nb;
+    for (int i = 0; i < 2; ++i) {
+        WMC(m->nodes[i].alloc(NB));
+        WMC(m->gs[i].alloc(SS));
+        WMC(m->gt[i].alloc(ST));
+    }
+    for (int i = 0; i < 3; ++i) {
+        WMC(m->hist[i].alloc(NB));
+        WMC(m->ls[i].alloc(SS));
+        WMC(m->lt[i].alloc(ST));
+    }
+    WMC(m->vel.alloc(NB));
+    WMC(m->velp.alloc(NB));
+    WMC(m->uind.alloc(NB));
+    WMC(m->gs1.alloc(SS));
+    WMC(m->gt1.alloc(ST));
+    WMC(m->rs.alloc(SS));
+    WMC(m->rt.alloc(ST));
+    WMC(m->rs1.alloc(SS));
+    WMC(m->rt1.alloc(ST));
+    WMC(m->segs.alloc(SEG));
+    WMC(m->llp.alloc((size_t)3 * NLL * sizeof(double)));
+    WMC(m->frozen.alloc((size_t)3 * NLL * sizeof(double)));
+    WMC(m->K.alloc((size_t)3 * NLL * NCH * sizeof(double)));
+    WMC(m->kjout.alloc((size_t)3 * NLL * sizeof(double)));
+    WMC(m->perf.alloc((size_t)5 * c.max_steps * sizeof(double)));
+    WMC(m->flag.alloc(256));
+    WMC(hipMemsetAsync(m->flag.p, 0, 256, m->st));
+    WMC(hipMemsetAsync(m->gt1.p, 0, ST, m->st));
+    for (int i = 0; i < 3; ++i) {
+        WMC(hipMemsetAsync(m->ls[i].p, 0, SS, m->st));
+        WMC(hipMemsetAsync(m->lt[i].p, 0, ST, m->st));
+    }
+#undef WMC
+    rc = setup(m);
+    if (rc) return fail(rc);
+    *out = m;
+    return WINDS_OK;
+}
+
+int winds_march_step(winds_march *m, int32_t nsteps) {
+    winds::clear_error();
+    if (!m) return set_error(WINDS_E_ARG, "null march");
+    for (int i = 0; i < nsteps; ++i) {
+        int rc = step_once(m);
+        if (rc) return rc;
+    }
+    return WINDS_OK;
+}
+
+int winds_march_sync(winds_march *m) {
+    winds::clear_error();
+    if (!m) return set_error(WINDS_E_ARG, "null march");
+    WM_HIP(hipStreamSynchronize(m->st));
+    int flag = 0;
+    WM_HIP(hipMemcpy(&flag, m->flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) return set_error(WINDS_E_ARG, "Diverging soln!!! (NaN lift coefficient in KJ)");
+    return WINDS_OK;
+}
+
+int winds_march_info(const winds_march *m, int64_t out[4]) {
+    if (!m || !out) return set_error(WINDS_E_ARG, "null argument");
+    out[0] = m->p;
+    out[1] = m->A;
+    out[2] = nodes_of(m, m->A);
+    out[3] = segs_of(m, m->A);
+    return WINDS_OK;
+}
+
+int winds_march_get_state(winds_march *m, double *nodes, double *gshed, double *gtrail) {
+    winds::clear_error();
+    if (!m) return set_error(WINDS_E_ARG, "null march");
+    WM_HIP(hipStreamSynchronize(m->st));
+    const int A = m->A, nst = m->g.nst, ns = m->g.ns, nb = m->g.nb;
+    const int64_t N = nodes_of(m, A);
+    if (nodes) {
+        std::vector<double> h((size_t)3 * N);
+        WM_HIP(hipMemcpy(h.data(), m->nodes[m->cur].p, h.size() * sizeof(double),
+                         hipMemcpyDeviceToHost));
+        // SoA over j = s + a*nst + b*nst*A  ->  [nst,3,A,nb] column-major
+        for (int b = 0; b < nb; ++b)
+            for (int a = 0; a < A; ++a)
+                for (int s = 0; s < nst; ++s) {
+                    const int64_t j = s + (int64_t)a * nst + (int64_t)b * nst * A;
+                    for (int c = 0; c < 3; ++c)
+                        nodes[s + (int64_t)c * nst + (int64_t)a * 3 * nst + (int64_t)b * 3 * nst * A] =
+                            h[c * N + j];
+                }
+    }
+    // circulation of the last completed time index (after its KJ solve)
+    const int Ag = m->A_gprev;
+    if (gshed)
+        WM_HIP(hipMemcpy(gshed, m->gs[m->gprev].p, (size_t)ns * Ag * nb * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    if (gtrail)
+        WM_HIP(hipMemcpy(gtrail, m->gt[m->gprev].p, (size_t)nst * (Ag - 1) * nb * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    return WINDS_OK;
+}
+
+int winds_march_set_state(winds_march *m, int32_t p, int32_t ages, const double *nodes,
+                          const double *gshed, const double *gtrail) {
+    winds::clear_error();
+    if (!m || !nodes || !gshed || !gtrail) return set_error(WINDS_E_ARG, "null argument");
+    if (ages < 3 || ages > m->cfg.max_ages || p < 4) return set_error(WINDS_E_DIMS, "bad ages/p");
+    WM_HIP(hipStreamSynchronize(m->st));
+    const int nst = m->g.nst, ns = m->g.ns, nb = m->g.nb, A = ages;
+    const int64_t N = nodes_of(m, A);
+    std::vector<double> h((size_t)3 * N);
+    for (int b = 0; b < nb; ++b)
+        for (int a = 0; a < A; ++a)
+            for (int s = 0; s < nst; ++s) {
+                const int64_t j = s + (int64_t)a * nst + (int64_t)b * nst * A;
+                for (int c = 0; c < 3; ++c)
+                    h[c * N + j] =
+                        nodes[s + (int64_t)c * nst + (int64_t)a * 3 * nst + (int64_t)b * 3 * nst * A];
+            }
+    // the wake given is time p; its circulation stands in for time p-1 (the
+    // Kelvin step of time p regenerates it from the bound values)
+    WM_HIP(hipMemcpy(m->nodes[m->cur].p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    WM_HIP(hipMemcpy(m->gs[m->gprev].p, gshed, (size_t)ns * (A - 1) * nb * sizeof(double),
+                     hipMemcpyHostToDevice));
+    WM_HIP(hipMemcpy(m->gt[m->gprev].p, gtrail, (size_t)nst * (A - 2) * nb * sizeof(double),
+                     hipMemcpyHostToDevice));
+    m->A_gprev = A - 1;
+    m->A = A;
+    m->p = p;
+    // lengths of time p-1: this wake's own lengths (zero strain on the first step)
+    const int64_t ns_ = segs_of(m, A);
+    k_vcore<<<blocks(ns_), 256, 0, m->st>>>(nst, nb, A, m->nodes[m->cur].d(), m->gs[m->gprev].d(),
+                                            m->gt[m->gprev].d(), m->r0, 0.0, 0, m->ls[0].d(),
+                                            m->lt[0].d(), A, m->ls[m->lcur].d(), m->lt[m->lcur].d(),
+                                            m->rs.d(), m->rt.d());
+    WM_HIP(hipGetLastError());
+    m->ls_A[m->lcur] = A;
+    return WINDS_OK;
+}
+
+int winds_march_get_perf(winds_march *m, int32_t first, int32_t n, double *rows) {
+    winds::clear_error();
+    if (!m || !rows) return set_error(WINDS_E_ARG, "null argument");
+    if (first < 0 || n < 0 || first + n > m->nsteps_done)
+        return set_error(WINDS_E_DIMS, "rows out of range");
+    WM_HIP(hipStreamSynchronize(m->st));
+    WM_HIP(hipMemcpy(rows, m->perf.d() + 5 * (int64_t)first, (size_t)5 * n * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    return WINDS_OK;
+}
+
+int winds_march_get_kj(winds_march *m, double *cl, double *cd, double *aoa) {
+    winds::clear_error();
+    if (!m) return set_error(WINDS_E_ARG, "null march");
+    WM_HIP(hipStreamSynchronize(m->st));
+    const size_t n = (size_t)m->g.ns * m->g.nb;
+    const double *k = m->kjout.d();
+    if (cl) WM_HIP(hipMemcpy(cl, k, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (cd) WM_HIP(hipMemcpy(cd, k + n, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (aoa) WM_HIP(hipMemcpy(aoa, k + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost));
+    return WINDS_OK;
+}
+
+const double *winds_march_uind_dev(const winds_march *m) { return m ? m->uind.d() : nullptr; }
+
+}  // extern "C"

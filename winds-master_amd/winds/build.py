@@ -23,6 +23,7 @@ SOURCES = [
     "bs_kernels.hip",
     "bs_api.cpp",
     "bs_batch.hip",
+    "march.hip",
 ]
 
 ARCH = "gfx950"

@@ -1,0 +1,145 @@
+"""Device-resident free-wake time march (include/winds_march.h) from Python."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+INTEG = {"pcc": 0, "fe": 1, "ab4": 2}
+
+
+class MarchCfg(C.Structure):
+    _fields_ = [
+        ("nb", C.c_int32), ("nst", C.c_int32), ("max_ages", C.c_int32), ("n_alpha", C.c_int32),
+        ("r_trail", C.c_void_p), ("chord_trail", C.c_void_p), ("twist_trail", C.c_void_p),
+        ("alpha", C.c_void_p), ("cl", C.c_void_p), ("cd", C.c_void_p),
+        ("R", C.c_double), ("tsr", C.c_double), ("omega", C.c_double), ("dt", C.c_double),
+        ("pitch_deg", C.c_double), ("uinf", C.c_double * 3), ("relax", C.c_double),
+        ("tol", C.c_double), ("maxiter", C.c_int32), ("integrator", C.c_int32),
+        ("max_steps", C.c_int32), ("reserved", C.c_int32),
+        ("wake", _lib.BsOpts), ("ll", _lib.BsOpts),
+    ]
+
+
+def _bind(L):
+    if getattr(L, "_winds_march_bound", False):
+        return L
+    P = C.c_void_p
+    D = C.POINTER(C.c_double)
+    L.winds_march_create.argtypes = [C.POINTER(MarchCfg), C.POINTER(C.c_void_p)]
+    L.winds_march_destroy.argtypes = [P]
+    L.winds_march_destroy.restype = None
+    L.winds_march_step.argtypes = [P, C.c_int32]
+    L.winds_march_sync.argtypes = [P]
+    L.winds_march_info.argtypes = [P, C.POINTER(C.c_int64)]
+    L.winds_march_get_state.argtypes = [P, D, D, D]
+    L.winds_march_set_state.argtypes = [P, C.c_int32, C.c_int32, D, D, D]
+    L.winds_march_get_perf.argtypes = [P, C.c_int32, C.c_int32, D]
+    L.winds_march_get_kj.argtypes = [P, D, D, D]
+    L.winds_march_uind_dev.argtypes = [P]
+    L.winds_march_uind_dev.restype = C.c_void_p
+    for f in ("create", "step", "sync", "info", "get_state", "set_state", "get_perf", "get_kj"):
+        getattr(L, "winds_march_" + f).restype = C.c_int
+    L._winds_march_bound = True
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+class March:
+    """Free-wake march of a rigid rotor (winds.rotor.Rotor) on one GPU."""
+
+    def __init__(self, rotor, max_ages: int, wake: _lib.BsOpts, ll: _lib.BsOpts | None = None,
+                 integrator: str = "pcc", relax: float = 0.25, tol: float = 1e-8,
+                 maxiter: int = 30, max_steps: int = 20000):
+        import torch  # the engine binds to torch's HIP runtime when torch is present
+        torch.cuda.init()
+        self.L = _bind(_lib.lib())
+        self.rotor = rotor
+        keep = [np.ascontiguousarray(x, dtype=np.float64) for x in
+                (rotor.r_trail, rotor.chord_trail, rotor.twist_trail, rotor.alpha, rotor.cl,
+                 rotor.cd)]
+        self._keep = keep
+        c = MarchCfg()
+        c.nb, c.nst, c.max_ages, c.n_alpha = rotor.nb, rotor.nst, max_ages, rotor.alpha.size
+        c.r_trail, c.chord_trail, c.twist_trail, c.alpha, c.cl, c.cd = [
+            x.ctypes.data for x in keep]
+        c.R, c.tsr, c.omega, c.dt = rotor.R, rotor.tsr, rotor.omega, rotor.dt
+        c.pitch_deg = rotor.pitch_deg
+        for i in range(3):
+            c.uinf[i] = float(rotor.uinf[i])
+        c.relax, c.tol, c.maxiter = relax, tol, maxiter
+        c.integrator = INTEG[integrator]
+        c.max_steps = max_steps
+        c.wake = wake
+        c.ll = ll if ll is not None else _lib.opts(
+            core=wake.core, vat_n=wake.vat_n, semantics=wake.semantics,
+            precision=_lib.PREC_FP64, d=wake.d, co=wake.co)
+        self.cfg = c
+        self._m = C.c_void_p()
+        _lib.check(self.L.winds_march_create(C.byref(c), C.byref(self._m)))
+
+    def __del__(self):
+        m = getattr(self, "_m", None)
+        if m is not None and m.value:
+            self.L.winds_march_destroy(m)
+            self._m = C.c_void_p()
+
+    def step(self, n: int = 1):
+        _lib.check(self.L.winds_march_step(self._m, int(n)))
+
+    def sync(self):
+        _lib.check(self.L.winds_march_sync(self._m))
+
+    def info(self) -> dict:
+        a = (C.c_int64 * 4)()
+        _lib.check(self.L.winds_march_info(self._m, a))
+        return {"p": a[0], "ages": a[1], "nodes": a[2], "segments": a[3]}
+
+    def state(self):
+        """(nodes [nst,3,A,nb] of the next time index, shed [ns,1,A',nb] and
+        trailing [nst,1,A'-1,nb] circulation of the last solved time index)."""
+        r = self.rotor
+        inf = self.info()
+        A = inf["ages"]
+        nodes = np.zeros((r.nst, 3, A, r.nb), order="F")
+        # circulation arrays are sized by the last solved time index
+        gs = np.zeros(r.ns * self.cfg.max_ages * r.nb)
+        gt = np.zeros(r.nst * self.cfg.max_ages * r.nb)
+        _lib.check(self.L.winds_march_get_state(self._m, _dp(nodes), _dp(gs), _dp(gt)))
+        return nodes, gs, gt
+
+    def state_circulation(self, ages_solved: int):
+        r = self.rotor
+        _, gs, gt = self.state()
+        A = ages_solved
+        gs = gs[: r.ns * A * r.nb].reshape((r.ns, 1, A, r.nb), order="F")
+        gt = gt[: r.nst * (A - 1) * r.nb].reshape((r.nst, 1, A - 1, r.nb), order="F")
+        return gs, gt
+
+    def set_state(self, p: int, nodes, gshed, gtrail):
+        nodes = np.asfortranarray(nodes, dtype=np.float64)
+        gshed = np.asfortranarray(gshed, dtype=np.float64)
+        gtrail = np.asfortranarray(gtrail, dtype=np.float64)
+        _lib.check(self.L.winds_march_set_state(self._m, int(p), int(nodes.shape[2]), _dp(nodes),
+                                                _dp(gshed), _dp(gtrail)))
+
+    def perf(self, first: int = 0, n: int | None = None) -> np.ndarray:
+        """Rows (p, T, Q, P, KJ iterations) of the stepped time indices."""
+        done = self.info()["p"] - 2
+        if n is None:
+            n = done - first
+        out = np.zeros((n, 5))
+        _lib.check(self.L.winds_march_get_perf(self._m, int(first), int(n), _dp(out)))
+        return out
+
+    def kj(self):
+        r = self.rotor
+        cl, cd, aoa = (np.zeros(r.ns * r.nb) for _ in range(3))
+        _lib.check(self.L.winds_march_get_kj(self._m, _dp(cl), _dp(cd), _dp(aoa)))
+        f = lambda a: a.reshape((r.ns, r.nb), order="F")
+        return f(cl), f(cd), f(aoa)
