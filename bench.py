@@ -137,6 +137,66 @@ def batch_bench(o32, o64, steps: int) -> dict:
             "tune": tune, "graph": True, "error_4_wakes_vs_fp64": worst}
 
 
+def march_opts(prec):
+    import winds
+    # WInDS defaults: cut-off core d = 1e-4 (WInDS.m:42) through the MEX path
+    return winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, precision=prec, d=1e-4)
+
+
+def march_bench(revs: int) -> dict:
+    """Free-wake time march of a 3-blade, 31-node rotor at 5 deg steps with
+    PCC (2 wake evaluations per step) and an FP64 Kutta-Joukowski solve:
+    (a) step time with the wake at its 1,440-age truncation length;
+    (b) thrust/power over `revs` revolutions from a cold start, FP32
+        compensated wake evaluations vs FP64."""
+    import numpy as np
+    import torch
+
+    import winds
+    from winds import gen, rotor as rmod
+    from winds.march import March
+
+    rot = rmod.synthetic_rotor(nb=3, nst=31)
+    out = {"rotor": "3 blades x 31 nodes, R 63 m, TSR 7, U 11.4 m/s, 5 deg steps, PCC",
+           "max_ages": 1440}
+    # (a) steady length: start from a 1,440-age helical wake
+    w = gen.rotor_wake(nb=3, nst=31, ages=1440, jitter=0.0, R=rot.R, hub=rot.hub, tsr=rot.tsr,
+                       U=rot.U, dpsi_deg=rot.dpsi_deg)
+    for name, prec, nsteps in (("fp32c", winds.PREC_FP32C, 10), ("fp64", winds.PREC_FP64, 2)):
+        m = March(rot, 1440, march_opts(prec), integrator="pcc")
+        m.set_state(1441, w.domain, w.gamma_shed[:, :, :-1], w.gamma_trail[:, :, :-1])
+        m.step(1)
+        m.sync()
+        t0 = time.perf_counter()
+        m.step(nsteps)
+        m.sync()
+        ms = (time.perf_counter() - t0) / nsteps * 1e3
+        out[f"step_ms_{name}"] = round(ms, 2)
+        log(f"march step at 1440 ages, {name} wake sums: {ms:.1f} ms")
+        del m
+    # (b) cold start, `revs` revolutions
+    steps = int(round(revs * 360 / rot.dpsi_deg))
+    hist = {}
+    for name, prec in (("fp32c", winds.PREC_FP32C), ("fp64", winds.PREC_FP64)):
+        t0 = time.perf_counter()
+        m = March(rot, 1440, march_opts(prec), integrator="pcc", max_steps=steps + 8)
+        m.step(steps)
+        m.sync()
+        hist[name] = m.perf()
+        log(f"march {revs} revs ({steps} steps) {name}: {time.perf_counter() - t0:.1f} s")
+        del m
+    a, b = hist["fp32c"], hist["fp64"]
+    last = slice(-int(360 / rot.dpsi_deg), None)   # final revolution
+    rel = lambda i: float(np.max(np.abs(a[last, i] - b[last, i]) / np.abs(b[last, i])))
+    out.update({"revs": revs, "steps": steps,
+                "thrust_N_last_rev_fp64": float(np.mean(b[last, 1])),
+                "power_W_last_rev_fp64": float(np.mean(b[last, 3])),
+                "thrust_max_rel_err_fp32c_vs_fp64_last_rev": rel(1),
+                "power_max_rel_err_fp32c_vs_fp64_last_rev": rel(3),
+                "kj_iterations_mean": float(np.mean(a[:, 4]))})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -144,6 +204,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-wake batch")
+    ap.add_argument("--no-march", action="store_true", help="skip the free-wake time march")
+    ap.add_argument("--march-revs", type=int, default=10)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 2 GiB device-to-device copy (reference point for "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -247,6 +309,7 @@ def main():
         del src, dst
 
     batch = None if args.no_batch else batch_bench(o32, o64, args.steps)
+    march = None if args.no_march else march_bench(args.march_revs)
 
     achieved = inter * FLOP_VATISTAS2 / (ms * 1e-3) / 1e12
     line = {
@@ -275,6 +338,7 @@ def main():
                        "tflops": round(inter * FLOP_CUTOFF / (ms_c * 1e-3) / 1e12, 2)},
         "fp64_ms_per_step": round(t64 * 1e3, 2),
         "batch_64_wakes": batch,
+        "free_wake_march": march,
         "roofline": {
             "bound": "valu",
             "achieved": round(achieved, 3),
