@@ -33,6 +33,10 @@ def test_every_declared_symbol_is_exported():
     assert len(march) >= 10
     for n in march:
         assert hasattr(L, n), n
+    mlp = header_functions("winds_mlp.h")
+    assert len(mlp) == 8
+    for n in mlp:
+        assert hasattr(L, n), n
 
 
 def test_version_and_defaults():

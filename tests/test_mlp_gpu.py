@@ -1,0 +1,121 @@
+"""BF16 MFMA network kernels (include/winds_mlp.h) against oracle/mlp.py.
+
+Tolerances: against the oracle with the device's bfloat16 roundings
+emulated, the forward output agrees to 1e-3 and each layer's gradient to
+1e-2 normwise (what is left is FP32 accumulation order and the rare bfloat16
+rounding it flips); against the plain FP64 network the gradients agree to
+5e-2 normwise (bfloat16 weights and activations)."""
+import numpy as np
+import pytest
+
+from conftest import flush_print
+from oracle import mlp as OM
+
+pytestmark = pytest.mark.gpu
+
+
+def make(n_in=40, hidden=64, layers=2, n_out=3, max_batch=256, seed=5):
+    from winds.mlp import MLP
+    return MLP(n_in, hidden, layers, n_out, max_batch, seed)
+
+
+def data(B, n_in, n_out, seed):
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(B, n_in)).astype(np.float32)
+    T = (0.3 * rng.normal(size=(B, n_out))).astype(np.float32)
+    return X, T
+
+
+def normwise(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+@pytest.mark.parametrize("B", [200, 64, 1])
+def test_forward_matches_oracle(need_gpu, B):
+    import torch
+    net = make()
+    flat = net.get_params()
+    params = [(W.astype(np.float64), b.astype(np.float64)) for W, b in net.unflatten(flat)]
+    X, _ = data(B, 40, 3, 1)
+    Y = net.forward(torch.from_numpy(X).cuda()).cpu().numpy()
+    Yo, _ = OM.forward(params, X, bf=True)
+    e = normwise(Y, Yo)
+    flush_print(f"[mlp forward B={B}] normwise {e:.2e}")
+    assert e <= 1e-3
+
+
+def test_gradients_match_oracle(need_gpu):
+    import torch
+    net = make()
+    # move the output layer off its small initial scale so every layer's
+    # gradient is well conditioned
+    flat = net.get_params()
+    rng = np.random.default_rng(2)
+    pairs = net.unflatten(flat)
+    pairs[-1][0][:] = rng.normal(0, 0.2, pairs[-1][0].shape)
+    pairs[-1][1][:] = rng.normal(0, 0.1, pairs[-1][1].shape)
+    net.set_params(flat)
+    assert np.array_equal(net.get_params(), flat)
+    params = [(W.astype(np.float64), b.astype(np.float64)) for W, b in net.unflatten(flat)]
+    X, T = data(200, 40, 3, 3)
+    g, loss = net.gradients(torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda())
+    lo, go = OM.loss_and_grads(params, X, T, bf=True)
+    l64, g64 = OM.loss_and_grads(params, X, T, bf=False)
+    assert abs(loss - lo) <= 1e-3 * lo
+    gd = net.unflatten(g)
+    for i, ((dW, db), (oW, ob), (fW, fb)) in enumerate(zip(gd, go, g64)):
+        eW, eb = normwise(dW, oW), normwise(db, ob)
+        fW_e = normwise(dW, fW)
+        flush_print(f"[mlp grad layer {i}] W {eW:.2e} b {eb:.2e} vs fp64 {fW_e:.2e}")
+        assert eW <= 1e-2 and eb <= 1e-2
+        assert fW_e <= 5e-2
+
+
+def test_first_adam_step_moves_by_lr_times_sign(need_gpu):
+    import torch
+    net = make(seed=9)
+    p0 = net.get_params()
+    X, T = data(128, 40, 3, 4)
+    Xd, Td = torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()
+    g, _ = net.gradients(Xd, Td)
+    lr = 1e-3
+    net.train_step(Xd, Td, lr)
+    torch.cuda.synchronize()
+    p1 = net.get_params()
+    big = np.abs(g) > 1e-4 * np.max(np.abs(g))
+    step = (p1 - p0)[big]
+    # Adam's first step is lr * g / (|g| + eps') = lr * sign(g) up to eps
+    assert np.allclose(step, -lr * np.sign(g[big]), rtol=1e-2, atol=1e-6)
+
+
+def test_training_fits_a_smooth_field(need_gpu):
+    import torch
+    from winds.mlp import MLP
+    net = MLP(4, 128, 3, 3, 1024, seed=1)
+    rng = np.random.default_rng(0)
+    X = rng.uniform(-1, 1, size=(8192, 4)).astype(np.float32)
+    T = np.stack([np.sin(2 * X[:, 0]) * X[:, 1], np.cos(X[:, 2] + X[:, 3]), X[:, 0] * X[:, 3]],
+                 axis=1).astype(np.float32)
+    Xd, Td = torch.from_numpy(X).cuda(), torch.from_numpy(T).cuda()
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    first = None
+    for it in range(600):
+        i = (it * 1024) % 8192
+        net.train_step(Xd[i:i + 1024], Td[i:i + 1024], 2e-3, loss)
+        if it == 0:
+            first = float(loss.item())
+    Y = torch.cat([net.forward(Xd[i:i + 1024]) for i in range(0, 8192, 1024)]).cpu().numpy()
+    mse = float(np.mean((Y - T) ** 2))
+    flush_print(f"[mlp train] first loss {first:.3e} final mse {mse:.3e}")
+    assert mse <= 0.02 * first
+
+
+def test_batch_limits(need_gpu):
+    import torch
+    import winds
+    net = make(max_batch=100)            # rounded up to 128 rows
+    X = torch.zeros((129, 40), dtype=torch.float32, device="cuda")
+    with pytest.raises(winds.WindsError):
+        net.forward(X)
+    Y = net.forward(X[:0])
+    assert Y.shape == (0, 3)

@@ -24,6 +24,7 @@ SOURCES = [
     "bs_api.cpp",
     "bs_batch.hip",
     "march.hip",
+    "mlp.hip",
 ]
 
 ARCH = "gfx950"
