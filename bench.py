@@ -26,6 +26,7 @@ for _p in (str(ROOT), str(ROOT / "winds-master_amd")):
 
 METRIC = "Biot–Savart G interactions/s & free-wake step ms vs FP64 plain, at stated err"
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA peak
 FP64_PEAK_TFLOPS = 78.6
 FLOP_VATISTAS2 = 59        # algorithmic FLOPs per interaction, Vatistas n=2 form
 FLOP_CUTOFF = 47           # cut-off (MEX) form
@@ -143,50 +144,225 @@ def march_opts(prec):
     return winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, precision=prec, d=1e-4)
 
 
-def march_bench(revs: int) -> dict:
-    """Free-wake time march of a 3-blade, 31-node rotor at 5 deg steps with
-    PCC (2 wake evaluations per step) and an FP64 Kutta-Joukowski solve:
-    (a) step time with the wake at its 1,440-age truncation length;
-    (b) thrust/power over `revs` revolutions from a cold start, FP32
-        compensated wake evaluations vs FP64."""
-    import numpy as np
+MLP_HIDDEN, MLP_LAYERS = 256, 4
+
+
+def mlp_weights(n_in: int, hidden: int, layers: int, n_out: int) -> int:
+    return n_in * hidden + (layers - 1) * hidden * hidden + hidden * n_out
+
+
+def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int = 4096):
+    """Far-wake network: direct-sum training data from seeded synthetic wakes
+    (TSR U[4,10], yaw U[0,30] deg, split at 144 ages = 2 revolutions) plus,
+    when given, direct sums from the engine's own march; training with the
+    engine's BF16 MFMA kernels; held-out error and inference rate.
+    Returns (report, network, in_norm, out_norm)."""
     import torch
 
     import winds
-    from winds import gen, rotor as rmod
+    from winds import surrogate as S
+    from winds.mlp import MLP
+
+    o = march_opts(winds.PREC_FP32C)
+    ts = S.build_training_set(n_samples, o, seed=7, log=log)
+    log(f"surrogate data: {ts.wakes} synthetic wakes, {ts.F.shape[0]} samples, "
+        f"{ts.interactions / 1e9:.1f} G interactions, {ts.seconds:.1f} s")
+    held = S.build_training_set(50_000, o, seed=8)
+    sets = [ts]
+    if march_set is not None:
+        # the march's own samples, repeated to weigh about as much as the synthetic ones
+        rep = max(1, int(round(0.5 * ts.F.shape[0] / max(1, march_set.F.shape[0]))))
+        sets += [march_set] * rep
+    full = S.concat(*sets)
+    in_norm, out_norm = S.fit_norm(full.F, full.Y)
+    Fn, Yn = S.normalise(full.F, full.Y, in_norm, out_norm)
+    Hn, _ = S.normalise(held.F, None, in_norm, out_norm)
+    rows = 31 * 108 * 3 + 30 * 3
+    net = MLP(S.NIN, MLP_HIDDEN, MLP_LAYERS, S.NOUT, max(batch, rows), seed=7)
+    secs, loss = S.train(net, Fn, Yn, steps=train_steps, batch=batch, lr=1e-3, log=log)
+    W = mlp_weights(S.NIN, MLP_HIDDEN, MLP_LAYERS, S.NOUT)
+    # training: forward 2W, weight gradients 2W, input gradients 2W (less the first layer)
+    train_flop = (6 * W - 2 * S.NIN * MLP_HIDDEN) * batch
+    train_tflops = train_flop * train_steps / secs / 1e12
+    Ft, _ = S.normalise(ts.F[:200_000], None, in_norm, out_norm)
+    err_train = S.normwise_error(S.predict(net, Ft, out_norm), ts.Y[:200_000])
+    err_held = S.normwise_error(S.predict(net, Hn, out_norm), held.Y)
+    err_march = None
+    if march_set is not None:
+        Fm, _ = S.normalise(march_set.F, None, in_norm, out_norm)
+        err_march = S.normwise_error(S.predict(net, Fm, out_norm), march_set.Y)
+    log(f"surrogate training: {train_steps} steps x {batch}: {secs:.1f} s "
+        f"({train_tflops:.1f} TFLOP/s), loss {loss:.3e}, normwise error synthetic train "
+        f"{err_train:.3e} held-out {err_held:.3e} march samples {err_march}")
+    # inference rate on max_batch rows
+    B = net.max_batch
+    Xb = Fn[:B].contiguous()
+    Yb = torch.empty((B, S.NOUT), dtype=torch.float32, device=Xb.device)
+    for _ in range(3):
+        net.forward(Xb, Yb)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 50
+    e0.record(st)
+    for _ in range(reps):
+        net.forward(Xb, Yb)
+    e1.record(st)
+    torch.cuda.synchronize()
+    fwd_ms = e0.elapsed_time(e1) / reps
+    inf_rate = B / (fwd_ms * 1e-3)
+    inf_tflops = 2 * W * B / (fwd_ms * 1e-3) / 1e12
+    log(f"surrogate inference: {B} rows in {fwd_ms:.3f} ms, {inf_rate / 1e6:.1f} M samples/s, "
+        f"{inf_tflops:.1f} TFLOP/s")
+    rep = {
+        "network": f"{S.NIN} inputs -> {MLP_LAYERS} x {MLP_HIDDEN} SiLU -> 3, BF16 weights and "
+                   f"activations, FP32 accumulation (v_mfma_f32_32x32x8bf16_1k)",
+        "parameters": net.num_params(),
+        "split_age": 144, "target_ages": 108,
+        "synthetic_samples": int(ts.F.shape[0]), "synthetic_wakes": ts.wakes,
+        "synthetic_data_G_interactions": round(ts.interactions / 1e9, 2),
+        "synthetic_data_s": round(ts.seconds, 2),
+        "march_samples": int(march_set.F.shape[0]) if march_set is not None else 0,
+        "march_snapshots": march_set.wakes if march_set is not None else 0,
+        "train_steps": train_steps, "batch": batch, "train_s": round(secs, 2),
+        "train_samples_per_s": round(train_steps * batch / secs, 1),
+        "train_tflops": round(train_tflops, 2),
+        "final_loss": loss,
+        "normwise_err_synthetic_train": err_train, "normwise_err_synthetic_heldout": err_held,
+        "normwise_err_march_samples": err_march,
+        "inference_rows": B, "inference_ms": round(fwd_ms, 4),
+        "inference_samples_per_s": round(inf_rate, 1),
+        "inference_tflops": round(inf_tflops, 2),
+        "roofline_inference": {"bound": "mfma", "achieved": round(inf_tflops, 2),
+                               "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                               "frac": round(inf_tflops / BF16_PEAK_TFLOPS, 4)},
+    }
+    return rep, net, in_norm, out_norm
+
+
+def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train_steps: int):
+    """Free-wake time march of a 3-blade, 31-node rotor at 5 deg steps with
+    PCC (2 wake evaluations per step), an FP64 Kutta-Joukowski solve and a
+    1,440-age wake:
+    (1) a cold start of dev_revs revolutions with FP32 compensated wake sums
+        (direct-sum snapshots of its far wake feed the surrogate's training);
+    (2) step time at the full length from that state: FP32, FP64, surrogate;
+    (3) `revs` more revolutions from that state, FP32 vs surrogate thrust/power;
+    (4) the first `revs` revolutions of the cold start, FP32 vs FP64.
+    Returns (march report, surrogate report)."""
+    import numpy as np
+
+    import winds
+    from winds import rotor as rmod
+    from winds import surrogate as S
     from winds.march import March
 
     rot = rmod.synthetic_rotor(nb=3, nst=31)
+    sc = S.MarchSurrogate()
+    per_rev = int(round(360 / rot.dpsi_deg))
     out = {"rotor": "3 blades x 31 nodes, R 63 m, TSR 7, U 11.4 m/s, 5 deg steps, PCC",
-           "max_ages": 1440}
-    # (a) steady length: start from a 1,440-age helical wake
-    w = gen.rotor_wake(nb=3, nst=31, ages=1440, jitter=0.0, R=rot.R, hub=rot.hub, tsr=rot.tsr,
-                       U=rot.U, dpsi_deg=rot.dpsi_deg)
-    for name, prec, nsteps in (("fp32c", winds.PREC_FP32C, 10), ("fp64", winds.PREC_FP64, 2)):
-        m = March(rot, 1440, march_opts(prec), integrator="pcc")
-        m.set_state(1441, w.domain, w.gamma_shed[:, :, :-1], w.gamma_trail[:, :, :-1])
-        m.step(1)
+           "max_ages": 1440, "develop_revs": dev_revs}
+    o32 = march_opts(winds.PREC_FP32C)
+    # (1) develop
+    t0 = time.perf_counter()
+    dev_steps = dev_revs * per_rev
+    m = March(rot, 1440, o32, integrator="pcc", max_steps=dev_steps + 8)
+    first = max(0, dev_steps - 10 * per_rev)
+    m.step(first)
+    m.sync()
+    log(f"march cold start: {first} steps in {time.perf_counter() - t0:.1f} s")
+    msamp = None
+    if surrogate:
+        msamp = S.march_training_set(m, o32, steps=dev_steps - first, every=24,
+                                     split=sc.split, target_ages=sc.target_ages, log=log)
+    else:
+        m.step(dev_steps - first)
         m.sync()
+    dev_s = time.perf_counter() - t0
+    log(f"march cold start {dev_revs} revs: {dev_s:.1f} s")
+    cold32 = m.perf()
+    inf = m.info()
+    nodes, _, _ = m.state()
+    A, Ag = inf["ages"], inf["ages_solved"]
+    gs, gt = m.state_circulation(Ag)
+    state = (inf["p"], nodes, gs[:, :, :A - 1], gt[:, :, :A - 2])
+    del m
+    out["develop_s_fp32c"] = round(dev_s, 2)
+
+    surr_rep, net_args = None, None
+    if surrogate:
+        surr_rep, net, in_norm, out_norm = surrogate_bench(n_samples, train_steps, msamp)
+        net_args = (net, in_norm, out_norm)
+
+    def fork(prec, with_surr, max_steps=64):
+        f = March(rot, 1440, march_opts(prec), integrator="pcc", max_steps=max_steps)
+        f.set_state(*state)
+        if with_surr:
+            f.set_surrogate(sc, *net_args)
+        return f
+
+    # (2) step time at the full length
+    runs = [("fp32c", winds.PREC_FP32C, 10, False), ("fp64", winds.PREC_FP64, 2, False)]
+    if surrogate:
+        runs.append(("surrogate", winds.PREC_FP32C, 2 * sc.check_every, True))
+    for name, prec, nsteps, ws in runs:
+        f = fork(prec, ws, max_steps=nsteps + 8)
+        f.step(1)
+        f.sync()
         t0 = time.perf_counter()
-        m.step(nsteps)
-        m.sync()
+        f.step(nsteps)
+        f.sync()
         ms = (time.perf_counter() - t0) / nsteps * 1e3
         out[f"step_ms_{name}"] = round(ms, 2)
-        log(f"march step at 1440 ages, {name} wake sums: {ms:.1f} ms")
-        del m
-    # (b) cold start, `revs` revolutions
-    steps = int(round(revs * 360 / rot.dpsi_deg))
-    hist = {}
-    for name, prec in (("fp32c", winds.PREC_FP32C), ("fp64", winds.PREC_FP64)):
-        t0 = time.perf_counter()
-        m = March(rot, 1440, march_opts(prec), integrator="pcc", max_steps=steps + 8)
-        m.step(steps)
-        m.sync()
-        hist[name] = m.perf()
-        log(f"march {revs} revs ({steps} steps) {name}: {time.perf_counter() - t0:.1f} s")
-        del m
-    a, b = hist["fp32c"], hist["fp64"]
-    last = slice(-int(360 / rot.dpsi_deg), None)   # final revolution
+        log(f"march step at 1440 ages, {name}: {ms:.1f} ms")
+        if ws:
+            out["surrogate_stats_steady"] = f.surr_stats()
+            out["surrogate_speedup_vs_fp32c"] = round(out["step_ms_fp32c"] / ms, 2)
+            log(f"  surrogate {out['surrogate_speedup_vs_fp32c']}x vs fp32c; "
+                f"{out['surrogate_stats_steady']}")
+        del f
+    # (3) revs more revolutions: plain FP32 vs surrogate
+    steps = revs * per_rev
+    if surrogate:
+        hist = {}
+        for name, ws in (("fp32c", False), ("surrogate", True)):
+            t0 = time.perf_counter()
+            f = fork(winds.PREC_FP32C, ws, max_steps=steps + 8)
+            done = 0
+            while done < steps:
+                k = min(per_rev, steps - done)
+                f.step(k)
+                f.sync()
+                done += k
+                log(f"  continued march {name}: {done}/{steps} steps")
+            hist[name] = f.perf()
+            el = time.perf_counter() - t0
+            out[f"continue_{revs}rev_s_{name}"] = round(el, 2)
+            log(f"march {revs} more revs ({steps} steps) {name}: {el:.1f} s")
+            if ws:
+                out["surrogate_stats_continue"] = f.surr_stats()
+            del f
+        a, b = hist["surrogate"], hist["fp32c"]
+        relh = lambda i: float(np.max(np.abs(a[:, i] - b[:, i]) / np.abs(b[:, i])))
+        meanr = lambda i: float(abs(np.mean(a[:, i]) - np.mean(b[:, i])) / abs(np.mean(b[:, i])))
+        out.update({
+            "surrogate_thrust_max_rel_err_vs_fp32c": relh(1),
+            "surrogate_power_max_rel_err_vs_fp32c": relh(3),
+            "surrogate_thrust_mean_rel_err_vs_fp32c": meanr(1),
+            "surrogate_power_mean_rel_err_vs_fp32c": meanr(3),
+            "thrust_N_mean_fp32c": float(np.mean(b[:, 1])),
+            "power_W_mean_fp32c": float(np.mean(b[:, 3])),
+        })
+        log(f"surrogate vs fp32c over {revs} revs: thrust max {relh(1):.2e} power max {relh(3):.2e}")
+    # (4) cold start: FP64 vs the FP32 cold start above
+    t0 = time.perf_counter()
+    m = March(rot, 1440, march_opts(winds.PREC_FP64), integrator="pcc", max_steps=steps + 8)
+    m.step(steps)
+    m.sync()
+    b = m.perf()
+    del m
+    log(f"march {revs} revs cold start fp64: {time.perf_counter() - t0:.1f} s")
+    a = cold32[:steps]
+    last = slice(-per_rev, None)
     rel = lambda i: float(np.max(np.abs(a[last, i] - b[last, i]) / np.abs(b[last, i])))
     out.update({"revs": revs, "steps": steps,
                 "thrust_N_last_rev_fp64": float(np.mean(b[last, 1])),
@@ -194,7 +370,7 @@ def march_bench(revs: int) -> dict:
                 "thrust_max_rel_err_fp32c_vs_fp64_last_rev": rel(1),
                 "power_max_rel_err_fp32c_vs_fp64_last_rev": rel(3),
                 "kj_iterations_mean": float(np.mean(a[:, 4]))})
-    return out
+    return out, surr_rep
 
 
 def main():
@@ -206,6 +382,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-wake batch")
     ap.add_argument("--no-march", action="store_true", help="skip the free-wake time march")
     ap.add_argument("--march-revs", type=int, default=10)
+    ap.add_argument("--develop-revs", type=int, default=20,
+                    help="cold-start revolutions before the full-length measurements")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the far-wake network")
+    ap.add_argument("--surrogate-samples", type=int, default=2_000_000)
+    ap.add_argument("--surrogate-steps", type=int, default=8000)
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 2 GiB device-to-device copy (reference point for "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -309,7 +490,10 @@ def main():
         del src, dst
 
     batch = None if args.no_batch else batch_bench(o32, o64, args.steps)
-    march = None if args.no_march else march_bench(args.march_revs)
+    march, surr_rep = None, None
+    if not args.no_march:
+        march, surr_rep = march_bench(args.march_revs, args.develop_revs, not args.no_surrogate,
+                                      args.surrogate_samples, args.surrogate_steps)
 
     achieved = inter * FLOP_VATISTAS2 / (ms * 1e-3) / 1e12
     line = {
@@ -338,6 +522,7 @@ def main():
                        "tflops": round(inter * FLOP_CUTOFF / (ms_c * 1e-3) / 1e12, 2)},
         "fp64_ms_per_step": round(t64 * 1e3, 2),
         "batch_64_wakes": batch,
+        "far_wake_surrogate": surr_rep,
         "free_wake_march": march,
         "roofline": {
             "bound": "valu",

@@ -28,6 +28,7 @@
 #include <stdint.h>
 
 #include "winds_bs.h"
+#include "winds_mlp.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -78,8 +79,9 @@ int winds_march_step(winds_march *m, int32_t nsteps);
  * (NaN lift coefficient, KuttaJoukowski.m:272-274) as an error. */
 int winds_march_sync(winds_march *m);
 
-/* out = {next time index p, ages, nodes, segments}. */
-int winds_march_info(const winds_march *m, int64_t out[4]);
+/* out = {next time index p, ages, nodes, segments, ages of the circulation
+ * of time p-1 (the last solved one), steps taken}. */
+int winds_march_info(const winds_march *m, int64_t out[6]);
 
 /* Copies to host, MATLAB layouts: nodes [nst,3,ages,nb]; shed circulation
  * [ns,1,ages,nb]; trailing circulation [nst,1,ages-1,nb]; each may be NULL. */
@@ -101,6 +103,48 @@ int winds_march_get_kj(winds_march *m, double *cl, double *cd, double *aoa);
 /* Total induced velocity of the last wake evaluation at every node, SoA
  * [3][nodes] in device memory (valid until the next step). */
 const double *winds_march_uind_dev(const winds_march *m);
+
+/* Far-wake surrogate (no counterpart in the reference, which sums the whole
+ * wake at every node, WInDS.m:150-154).  Once the wake is longer than split
+ * ages, a step evaluates
+ *   near targets (nodes of age < target_ages) and the lifting line:
+ *       direct sum over the near segments (both endpoints younger than split)
+ *       + the far segments' velocity from the network (WINDS_SURR_MLP), the
+ *       direct far sum (WINDS_SURR_DIRECT) or nothing (WINDS_SURR_DROP);
+ *   older nodes: the induced velocity of the node of the same age at the
+ *       last check, held in the frame turning with the rotor.
+ * Every check_every steps (and on the first such step) the step is a full
+ * direct sum; it refreshes the held velocities and, in WINDS_SURR_MLP,
+ * compares the network with the direct far sum at the near targets and the
+ * lifting line (normwise: max |u_net - u_far| / max |u_far|, each point set
+ * separately).  A check above tol switches the following steps to the
+ * direct far sum until a check passes; train_iters Adam steps (rate lr) on
+ * the check's direct sums follow every check.  The network needs
+ * WINDS_SURR_NIN inputs, 3 outputs and max_batch >= nst*target_ages*nb +
+ * (nst-1)*nb; it is not owned and must outlive its use.  s == NULL turns the
+ * surrogate off. */
+#define WINDS_SURR_MLP 0
+#define WINDS_SURR_DIRECT 1
+#define WINDS_SURR_DROP 2
+
+typedef struct winds_march_surr {
+    int32_t mode;
+    int32_t split;          /* first far age, 2 <= split < max_ages          */
+    int32_t target_ages;    /* 1 <= target_ages <= split                      */
+    int32_t check_every;    /* >= 1                                           */
+    int32_t train_iters;    /* >= 0                                           */
+    int32_t reserved;
+    double tol;             /* fallback threshold, e.g. 0.01                  */
+    double lr;              /* Adam rate of the online training               */
+    const float *in_norm;   /* host [2][WINDS_SURR_NIN]: mean, 1/std          */
+    const float *out_norm;  /* host [2][3]: mean, std                         */
+} winds_march_surr;
+
+int winds_march_set_surrogate(winds_march *m, winds_mlp *net, const winds_march_surr *s);
+
+/* out = {checks, failed checks, network steps, direct-far steps, last node
+ * error, last lifting-line error, largest error, Adam steps}. */
+int winds_march_surr_stats(const winds_march *m, double out[8]);
 
 #ifdef __cplusplus
 }

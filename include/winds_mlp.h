@@ -31,19 +31,27 @@ void winds_mlp_destroy(winds_mlp *m);
 
 /* Number of parameters of the flat host layout. */
 int64_t winds_mlp_num_params(const winds_mlp *m);
+
+/* out = {n_in, n_hidden, n_hidden_layers, n_out, max_batch (rounded up to a
+ * multiple of 64), parameters}. */
+int winds_mlp_info(const winds_mlp *m, int64_t out[6]);
 int winds_mlp_get_params(winds_mlp *m, float *host);
 int winds_mlp_set_params(winds_mlp *m, const float *host);
 
-/* Y = net(X) for B rows (any B <= max_batch). */
+/* Y = net(X) for B rows (any B <= max_batch), enqueued on stream (NULL: the
+ * default stream).  Calls on one network share its activation buffers, so
+ * they must not run concurrently on different streams. */
 int winds_mlp_forward(winds_mlp *m, const float *X, int32_t B, float *Y, void *stream);
 
 /* Mean-squared-error gradients of all parameters for one batch (no update),
- * copied to host in the flat layout; *loss receives the batch loss. */
+ * copied to host in the flat layout; *loss receives the batch loss.  Runs on
+ * the network's own stream: X and T must be ready when it is called. */
 int winds_mlp_gradients(winds_mlp *m, const float *X, const float *T, int32_t B, float *grads,
                         float *loss);
 
-/* One Adam step on the MSE loss of the batch (B <= max_batch).  loss (device
- * FP32 scalar, may be NULL) receives the batch loss before the update. */
+/* One Adam step on the MSE loss of the batch (B <= max_batch) on stream
+ * (NULL: the default stream).  loss (device FP32 scalar, may be NULL)
+ * receives the batch loss before the update. */
 int winds_mlp_train_step(winds_mlp *m, const float *X, const float *T, int32_t B, float lr,
                          float *loss, void *stream);
 

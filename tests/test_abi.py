@@ -34,8 +34,12 @@ def test_every_declared_symbol_is_exported():
     for n in march:
         assert hasattr(L, n), n
     mlp = header_functions("winds_mlp.h")
-    assert len(mlp) == 8
+    assert len(mlp) == 9
     for n in mlp:
+        assert hasattr(L, n), n
+    sur = header_functions("winds_surrogate.h")
+    assert len(sur) == 3
+    for n in sur:
         assert hasattr(L, n), n
 
 

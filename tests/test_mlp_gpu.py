@@ -110,6 +110,22 @@ def test_training_fits_a_smooth_field(need_gpu):
     assert mse <= 0.02 * first
 
 
+def test_calls_follow_the_default_stream(need_gpu):
+    """Inputs written by torch on the default stream just before the call are
+    what the network reads, and its output is complete when torch reads it."""
+    import torch
+    net = make(max_batch=4096)
+    params = [(W.astype(np.float64), b.astype(np.float64)) for W, b in net.unflatten(net.get_params())]
+    torch.manual_seed(0)
+    worst = 0.0
+    for k in range(8):
+        X = torch.randn(4096, 40, device="cuda") * (1.0 + k)
+        Y = net.forward(X)
+        Yo, _ = OM.forward(params, X.cpu().numpy(), bf=True)
+        worst = max(worst, normwise(Y.cpu().numpy(), Yo))
+    assert worst <= 1e-3
+
+
 def test_batch_limits(need_gpu):
     import torch
     import winds

@@ -1,0 +1,213 @@
+"""Far-wake surrogate: network inputs/outputs and the surrogate march.
+
+Inputs are checked against a numpy restatement of their definition
+(include/winds_surrogate.h); the march is checked (a) to reduce exactly to
+the plain march when every step is a check, (b) to stay within 1% of the
+plain march's thrust and power with the direct far sum and the held far-node
+velocities, and (c) to gain over dropping the far wake when the network
+supplies the far wake's velocity."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import flush_print
+import winds
+from winds import gen, rotor as rmod
+
+pytestmark = pytest.mark.gpu
+
+
+def np_features(w, split, max_ages, R, U, psi, P):
+    nst, A, nb = w.nst, w.ages, w.nb
+    gt = np.asarray(w.gamma_trail)[:, 0]            # [nst, A-1, nb]
+    d = w.domain
+    width = max(1, -(-(max_ages - split) // 8))
+    desc = []
+    for k in range(8):
+        lo, hi = split + k * width, min(split + (k + 1) * width, A - 1)
+        if hi <= lo:
+            desc += [0.0] * 5
+            continue
+        tip = d[nst - 1, :, lo:hi, :]                # [3, na, nb]
+        desc += [gt[nst - 1, lo:hi].mean() * 10 / (U * R), gt[0, lo:hi].mean() * 10 / (U * R),
+                 tip[0].mean() / R, np.sqrt(tip[1] ** 2 + tip[2] ** 2).mean() / R,
+                 tip[1].mean() / R]
+    desc += [math.cos(psi), math.sin(psi), math.cos(nb * psi), math.sin(nb * psi),
+             min(1.0, max(0.0, (A - 1 - split) / max(1, max_ages - 1 - split)))]
+    x, y, z = P
+    c, s = math.cos(psi), math.sin(psi)
+    tf = np.stack([x / R, (y * c + z * s) / R, (-y * s + z * c) / R, np.sqrt(y * y + z * z) / R], 1)
+    return np.concatenate([tf, np.broadcast_to(np.array(desc), (P.shape[1], len(desc)))], 1)
+
+
+def test_features_labels_apply(need_gpu):
+    import torch
+    from winds import surrogate as S
+    w = gen.rotor_wake(nb=3, nst=9, ages=70, R=30.0, hub=1.0, tsr=6.0, U=9.0, jitter=0.01,
+                       yaw_deg=15.0, psi0=0.7)
+    split, max_ages, psi = 30, 95, 0.7
+    X, gt = S.march_layout(w)
+    P = S.query_points(w, 20)
+    g = S.geom(3, 9, 70, split, max_ages, 30.0, 9.0, psi)
+    Xd, gtd, Pd = (torch.from_numpy(a).cuda() for a in (X, gt, P))
+    F = S.features(g, Xd, gtd, Pd).cpu().numpy().astype(np.float64)
+    Fo = np_features(w, split, max_ages, 30.0, 9.0, psi, P)
+    assert F.shape == Fo.shape == (P.shape[1], S.NIN)
+    assert np.allclose(F, Fo, rtol=2e-6, atol=2e-6)
+    # normalisation
+    inn = np.stack([np.linspace(-1, 1, S.NIN), np.linspace(0.5, 2, S.NIN)]).astype(np.float32)
+    Fn = S.features(g, Xd, gtd, Pd, torch.from_numpy(inn).cuda()).cpu().numpy()
+    assert np.allclose(Fn, (F - inn[0]) * inn[1], rtol=1e-5, atol=1e-5)
+    # labels -> apply is the identity on a velocity field (FP32 rounding)
+    u = torch.from_numpy(np.random.default_rng(1).normal(size=(3, P.shape[1]))).cuda()
+    on = torch.tensor([[0.1, -0.2, 0.3], [2.0, 0.5, 1.5]], dtype=torch.float32).cuda()
+    T = S.labels(g, u, on)
+    back = S.apply(g, T, torch.zeros_like(u), on)
+    assert torch.allclose(back, u, rtol=1e-5, atol=1e-5 * 9.0)
+    # turning frame: the axial component is unrotated, the others rotate by psi
+    Tr = S.labels(g, u).cpu().numpy()
+    un = u.cpu().numpy() / 9.0
+    c, s = math.cos(psi), math.sin(psi)
+    assert np.allclose(Tr[:, 0], un[0], rtol=1e-6, atol=1e-7)
+    assert np.allclose(Tr[:, 1], un[1] * c + un[2] * s, rtol=1e-5, atol=1e-6)
+
+
+def small_rotor():
+    return rmod.synthetic_rotor(nb=2, nst=7, R=20.0, hub=1.0, tsr=6.0, U=10.0, dpsi_deg=10.0)
+
+
+def march(rot, max_ages, prec=winds.PREC_FP64):
+    from winds.march import March
+    o = winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, precision=prec, d=1e-4)
+    return March(rot, max_ages, o, integrator="pcc")
+
+
+def test_checks_every_step_reduce_to_the_plain_march(need_gpu):
+    from winds.surrogate import MarchSurrogate
+    rot = small_rotor()
+    a, b = march(rot, 60), march(rot, 60)
+    b.set_surrogate(MarchSurrogate(mode=1, split=20, target_ages=15, check_every=1))
+    for m in (a, b):
+        m.step(50)
+        m.sync()
+    pa, pb = a.perf(), b.perf()
+    assert np.array_equal(pa, pb)
+    st = b.surr_stats()
+    assert st["checks"] >= 25 and st["direct_far_steps"] == 0 and st["network_steps"] == 0
+
+
+def rel_perf_err(p, ref):
+    return float(np.max(np.abs(p[:, 1:4] - ref[:, 1:4]) / np.abs(ref[:, 1:4]).max(0)))
+
+
+def test_direct_far_with_held_far_velocities_within_one_percent(need_gpu):
+    from winds.surrogate import MarchSurrogate
+    rot = small_rotor()
+    steps = 36 * 10
+    a, b, c = march(rot, 200), march(rot, 200), march(rot, 200)
+    b.set_surrogate(MarchSurrogate(mode=1, split=72, target_ages=54, check_every=18))
+    c.set_surrogate(MarchSurrogate(mode=2, split=72, target_ages=54, check_every=18))
+    for m in (a, b, c):
+        m.step(steps)
+        m.sync()
+    pa, pb, pc = a.perf(), b.perf(), c.perf()
+    eb, ec = rel_perf_err(pb, pa), rel_perf_err(pc, pa)
+    st = b.surr_stats()
+    flush_print(f"[surrogate direct-far] T/Q/P err {eb:.2e}; far wake dropped {ec:.2e}; {st}")
+    assert st["direct_far_steps"] > 0.8 * (steps - 72)
+    assert eb <= 0.01
+    assert ec > eb
+
+
+def fork(state, rot, max_ages):
+    p, nodes, gs, gt = state
+    m = march(rot, max_ages)
+    m.set_state(p, nodes, gs, gt)
+    return m
+
+
+@pytest.fixture(scope="module")
+def developed():
+    """The small rotor marched 6 revolutions with a 200-age wake; its state,
+    and direct-sum samples of its far wake over revolutions 4-6."""
+    from winds import surrogate as S
+    rot = small_rotor()
+    o32 = winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, precision=winds.PREC_FP32C,
+                     d=1e-4)
+    m = march(rot, 200)
+    m.step(108)
+    ts = S.march_training_set(m, o32, steps=108, every=3, split=72, target_ages=54)
+    inf = m.info()
+    nodes, _, _ = m.state()
+    A, Ag = inf["ages"], inf["ages_solved"]
+    gs, gt = m.state_circulation(Ag)
+    gs, gt = gs[:, :, :A - 1], gt[:, :, :A - 2]
+    return rot, (inf["p"], nodes, gs, gt), ts
+
+
+@pytest.fixture(scope="module")
+def small_net(developed):
+    """A network trained on synthetic wakes shaped like small_rotor() and on
+    the developed march's own direct sums."""
+    import torch
+    from winds import surrogate as S
+    from winds.mlp import MLP
+    _, _, tm = developed
+    o = winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, precision=winds.PREC_FP32C,
+                   d=1e-4)
+    tg = S.build_training_set(100_000, o, seed=7, split=72, target_ages=54, max_ages=200, nb=2,
+                              R=20.0, U=10.0, hub=1.0, nst_range=(7, 7), dpsi_deg=10.0)
+    ts = S.concat(tg, *([tm] * 5))
+    in_norm, out_norm = S.fit_norm(ts.F, ts.Y)
+    Fn, Yn = S.normalise(ts.F, ts.Y, in_norm, out_norm)
+    net = MLP(S.NIN, 256, 4, S.NOUT, 2048, seed=7)
+    secs, loss = S.train(net, Fn, Yn, steps=8000, batch=1024, lr=1e-3)
+    Fm, _ = S.normalise(tm.F, None, in_norm, out_norm)
+    err = S.normwise_error(S.predict(net, Fm, out_norm), tm.Y)
+    flush_print(f"[surrogate pretrain] {tg.wakes} synthetic wakes + {tm.wakes} march snapshots, "
+                f"{Fn.shape[0]} samples, {secs:.1f} s, loss {loss:.3e}, normwise error on the "
+                f"march samples {err:.3e}")
+    return net, in_norm, out_norm, err
+
+
+def test_trained_network_fits_the_march_samples(need_gpu, small_net):
+    assert small_net[3] <= 0.03
+
+
+def test_network_gains_over_dropping_the_far_wake(need_gpu, developed, small_net):
+    from winds.surrogate import MarchSurrogate
+    net, in_norm, out_norm, _ = small_net
+    rot, state, _ = developed
+    steps = 36 * 10
+    a, b, c = (fork(state, rot, 200) for _ in range(3))
+    # tol far above any error: the network is used on every surrogate step
+    b.set_surrogate(MarchSurrogate(mode=0, split=72, target_ages=54, check_every=18,
+                                   train_iters=20, tol=1e9, lr=1e-4), net, in_norm, out_norm)
+    c.set_surrogate(MarchSurrogate(mode=2, split=72, target_ages=54, check_every=18))
+    for m in (a, b, c):
+        m.step(steps)
+        m.sync()
+    pa, pb, pc = a.perf(), b.perf(), c.perf()
+    eb, ec = rel_perf_err(pb, pa), rel_perf_err(pc, pa)
+    st = b.surr_stats()
+    flush_print(f"[surrogate network] T/Q/P err {eb:.2e}; far wake dropped {ec:.2e}; {st}")
+    assert st["network_steps"] > 0.8 * steps and st["adam_steps"] > 0
+    assert eb < 0.5 * ec
+    assert eb <= 0.01
+
+
+def test_failed_check_falls_back_to_the_direct_sum(need_gpu, developed, small_net):
+    from winds.surrogate import MarchSurrogate
+    net, in_norm, out_norm, _ = small_net
+    rot, state, _ = developed
+    m = fork(state, rot, 200)
+    # a tolerance no network meets: every check fails, every step is direct
+    m.set_surrogate(MarchSurrogate(mode=0, split=72, target_ages=54, check_every=18,
+                                   train_iters=0, tol=1e-12), net, in_norm, out_norm)
+    m.step(80)
+    m.sync()
+    st = m.surr_stats()
+    assert st["checks"] >= 4 and st["failed_checks"] == st["checks"]
+    assert st["network_steps"] == 0 and st["direct_far_steps"] > 0
+    assert 0.0 < st["last_err_nodes"] < 1.0

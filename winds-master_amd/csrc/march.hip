@@ -21,7 +21,9 @@
 #include "bs_common.hpp"
 #include "bs_kernels.hpp"
 #include "bs_pair.hpp"
+#include "surrogate.hpp"
 #include "winds_march.h"
+#include "winds_mlp.h"
 
 namespace winds {
 namespace mk {
@@ -253,34 +255,39 @@ __global__ void k_vcore(int nst, int nb, int A, const double *__restrict__ X,
     }
 }
 
-// Segment records from nodes and circulation (shed first, then trailing).
+// Segment records from nodes and circulation (shed first, then trailing),
+// for shed ages [s0, s1) and trailing ages [t0, t1) of every blade.
 // zero_changing zeroes the circulation of the filaments the Kutta-Joukowski
 // iteration changes: shed ages 0 and 1, trailing age 0.
 __global__ void k_segments(int nst, int nb, int A, const double *__restrict__ X,
                            const double *__restrict__ gs, const double *__restrict__ gt,
                            const double *__restrict__ rs, const double *__restrict__ rt,
-                           int zero_changing, Seg64 *__restrict__ out) {
+                           int zero_changing, int s0, int s1, int t0, int t1,
+                           Seg64 *__restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int ns = nst - 1;
+    const int ws = s1 - s0, wt = t1 - t0;
     const int64_t N = (int64_t)nst * A * nb;
-    const int64_t n_s = (int64_t)ns * A * nb;
-    const int64_t n_t = (int64_t)nst * (A - 1) * nb;
+    const int64_t n_s = (int64_t)ns * ws * nb;
+    const int64_t n_t = (int64_t)nst * wt * nb;
     if (i >= n_s + n_t) return;
     Seg64 s;
     int64_t j1, j2;
     if (i < n_s) {
         const int m = (int)(i % ns);
-        const int a = (int)((i / ns) % A);
-        const int b = (int)(i / ((int64_t)ns * A));
+        const int a = s0 + (int)((i / ns) % ws);
+        const int b = (int)(i / ((int64_t)ns * ws));
+        const int64_t k = m + (int64_t)a * ns + (int64_t)b * ns * A;
         j1 = m + (int64_t)a * nst + (int64_t)b * nst * A;
         j2 = j1 + 1;
-        s.g = (zero_changing && a <= 1) ? 0.0 : gs[i];
-        s.rc = rs[i];
+        s.g = (zero_changing && a <= 1) ? 0.0 : gs[k];
+        s.rc = rs[k];
     } else {
-        const int64_t k = i - n_s;
-        const int m = (int)(k % nst);
-        const int a = (int)((k / nst) % (A - 1));
-        const int b = (int)(k / ((int64_t)nst * (A - 1)));
+        const int64_t q = i - n_s;
+        const int m = (int)(q % nst);
+        const int a = t0 + (int)((q / nst) % wt);
+        const int b = (int)(q / ((int64_t)nst * wt));
+        const int64_t k = m + (int64_t)a * nst + (int64_t)b * nst * (A - 1);
         j2 = m + (int64_t)a * nst + (int64_t)b * nst * A;
         j1 = j2 + nst;
         s.g = (zero_changing && a == 0) ? 0.0 : gt[k];
@@ -293,6 +300,96 @@ __global__ void k_segments(int nst, int nb, int A, const double *__restrict__ X,
     s.by = X[N + j2];
     s.bz = X[2 * N + j2];
     out[i] = s;
+}
+
+// Near-wake targets: the nodes of age < St, compacted blade-major into
+// SoA [3][nst*St*nb].
+__global__ void k_gather_near(int nst, int nb, int A, int St, const double *__restrict__ X,
+                              double *__restrict__ T) {
+    const int64_t n = (int64_t)nst * St * nb;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int m = (int)(i % nst);
+    const int a = (int)((i / nst) % St);
+    const int b = (int)(i / ((int64_t)nst * St));
+    const int64_t N = (int64_t)nst * A * nb;
+    const int64_t j = m + (int64_t)a * nst + (int64_t)b * nst * A;
+    T[i] = X[j];
+    T[n + i] = X[N + j];
+    T[2 * n + i] = X[2 * N + j];
+}
+
+// Induced velocity at every node: nodes of age < St from unear (compacted as
+// k_gather_near), older nodes from the store taken at the last direct
+// evaluation, kept in the frame turning with the rotor (cos/sin of the
+// blade-0 azimuth c, s) and indexed by age (ages beyond the store's Aref use
+// its oldest age).
+__global__ void k_compose(int nst, int nb, int A, int St, const double *__restrict__ unear,
+                          const double *__restrict__ ufz, int Aref, double c, double s,
+                          double *__restrict__ u) {
+    const int64_t N = (int64_t)nst * A * nb;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const int m = (int)(j % nst);
+    const int a = (int)((j / nst) % A);
+    const int b = (int)(j / ((int64_t)nst * A));
+    if (a < St) {
+        const int64_t n = (int64_t)nst * St * nb;
+        const int64_t i = m + (int64_t)a * nst + (int64_t)b * nst * St;
+        u[j] = unear[i];
+        u[N + j] = unear[n + i];
+        u[2 * N + j] = unear[2 * n + i];
+    } else {
+        const int ar = min(a, Aref - 1);
+        const int64_t Nr = (int64_t)nst * Aref * nb;
+        const int64_t k = m + (int64_t)ar * nst + (int64_t)b * nst * Aref;
+        const double vy = ufz[Nr + k], vz = ufz[2 * Nr + k];
+        u[j] = ufz[k];
+        u[N + j] = vy * c - vz * s;
+        u[2 * N + j] = vy * s + vz * c;
+    }
+}
+
+// ufz = the induced velocity u in the frame turning with the rotor
+__global__ void k_store_turning(int64_t N, double c, double s, const double *__restrict__ u,
+                                double *__restrict__ ufz) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    const double uy = u[N + j], uz = u[2 * N + j];
+    ufz[j] = u[j];
+    ufz[N + j] = uy * c + uz * s;
+    ufz[2 * N + j] = -uy * s + uz * c;
+}
+
+__global__ void k_accumulate(int64_t n, const double *__restrict__ a, double *__restrict__ u) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] += a[i];
+}
+
+__global__ void k_zero(int64_t n, double *__restrict__ u) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] = 0.0;
+}
+
+// out[0] = max_i |a_i - b_i|, out[1] = max_i |b_i| over points (SoA [3][n]);
+// non-negative doubles order like their bit patterns.
+__global__ void k_max_err(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+                          unsigned long long *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    double e = 0.0, r = 0.0;
+    if (i < n) {
+        const double dx = a[i] - b[i], dy = a[n + i] - b[n + i], dz = a[2 * n + i] - b[2 * n + i];
+        e = sqrt(dx * dx + dy * dy + dz * dz);
+        r = sqrt(b[i] * b[i] + b[n + i] * b[n + i] + b[2 * n + i] * b[2 * n + i]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        e = fmax(e, __shfl_down(e, off));
+        r = fmax(r, __shfl_down(r, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(out, (unsigned long long)__double_as_longlong(e));
+        atomicMax(out + 1, (unsigned long long)__double_as_longlong(r));
+    }
 }
 
 __global__ void k_add_uinf(int64_t n, double ux, double uy, double uz,
@@ -633,6 +730,16 @@ struct winds_march {
     DBuf gs[2], gt[2], gs1, gt1, rs, rt, rs1, rt1, ls[3], lt[3];
     int ls_A[3] = {0, 0, 0};
     DBuf segs, ws, llp, frozen, K, kjout, perf, flag;
+    // far-wake surrogate (winds_march_set_surrogate); net is not owned
+    double umag = 0.0;
+    winds_mlp *net = nullptr;
+    winds_march_surr sc{};
+    bool surr = false;
+    bool fallback = true;  // use the direct far sum until a check passes
+    int A_ref = 0;         // ages of the turning-frame store ufz (0: empty)
+    int64_t since_check = 0;
+    DBuf norm, tgt, unear, ufar, umlp, ufz, llfar, llmlp, feat, lab, yout, desc, err;
+    double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -660,18 +767,52 @@ int ensure_ws(winds_march *m, int64_t nt, int64_t nseg, const winds::EvalCfg &c)
     return WINDS_OK;
 }
 
-// Induced velocity at targets t3 (nt) of the wake (nodes, circulation, cores)
-int induced(winds_march *m, const double *X, int A, const double *gsp, const double *gtp,
-            const double *rsp, const double *rtp, const double *t3, int64_t nt, double *u3,
-            const winds::EvalCfg &c, int zero_changing) {
-    const int64_t ns = segs_of(m, A);
-    k_segments<<<blocks(ns), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, X, gsp, gtp, rsp, rtp,
-                                              zero_changing, (winds::Seg64 *)m->segs.p);
+enum Part { kAll = 0, kNear = 1, kFar = 2 };
+
+// Pack the segments of one part of the wake into m->segs; returns their count.
+// Near: both endpoints younger than the split age S (shed ages < S, trailing
+// ages < S-1); far: the others.
+int64_t pack_segments(winds_march *m, const double *X, int A, const double *gsp,
+                      const double *gtp, const double *rsp, const double *rtp, int zero_changing,
+                      Part part) {
+    const int S = m->sc.split;
+    int s0 = 0, s1 = A, t0 = 0, t1 = A - 1;
+    if (part == kNear) {
+        s1 = std::min(S, A);
+        t1 = std::min(S - 1, A - 1);
+    } else if (part == kFar) {
+        s0 = std::min(S, A);
+        t0 = std::min(S - 1, A - 1);
+    }
+    const int64_t n = (int64_t)m->g.ns * (s1 - s0) * m->g.nb + (int64_t)m->g.nst * (t1 - t0) * m->g.nb;
+    if (n > 0)
+        k_segments<<<blocks(n), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, X, gsp, gtp, rsp, rtp,
+                                                 zero_changing, s0, s1, t0, t1,
+                                                 (winds::Seg64 *)m->segs.p);
+    return n;
+}
+
+// Induced velocity at targets t3 (nt) of the segments last packed (ns)
+int eval_packed(winds_march *m, int64_t ns, const double *t3, int64_t nt, double *u3,
+                const winds::EvalCfg &c) {
     WM_HIP(hipGetLastError());
+    if (ns == 0) {
+        k_zero<<<blocks(3 * nt), 256, 0, m->st>>>(3 * nt, u3);
+        WM_HIP(hipGetLastError());
+        return WINDS_OK;
+    }
     int rc = ensure_ws(m, nt, ns, c);
     if (rc) return rc;
     return winds::eval_dev(t3, nt, (const winds::Seg64 *)m->segs.p, ns, c, u3, m->ws.p,
                            m->ws.bytes, m->st);
+}
+
+// Induced velocity at targets t3 (nt) of the wake (nodes, circulation, cores)
+int induced(winds_march *m, const double *X, int A, const double *gsp, const double *gtp,
+            const double *rsp, const double *rtp, const double *t3, int64_t nt, double *u3,
+            const winds::EvalCfg &c, int zero_changing, Part part = kAll) {
+    const int64_t ns = pack_segments(m, X, A, gsp, gtp, rsp, rtp, zero_changing, part);
+    return eval_packed(m, ns, t3, nt, u3, c);
 }
 
 int kelvin(winds_march *m, const double *gs_old, int A_old, const double *gt_old, int A_new,
@@ -735,6 +876,147 @@ int push_hist(winds_march *m, const double *vel, int A) {
     return WINDS_OK;
 }
 
+// ---- far-wake surrogate -------------------------------------------------
+
+bool surr_on(const winds_march *m, int A) { return m->surr && A > m->sc.split; }
+
+double psi_of(const winds_march *m, double t) { return m->cfg.omega * t; }
+
+winds::surr::Geo geo_of(const winds_march *m, int A, double t) {
+    return winds::surr::Geo{m->g.nb, m->g.nst, A, m->sc.split, m->cfg.max_ages, m->cfg.R,
+                            m->umag, psi_of(m, t)};
+}
+
+int64_t near_targets(const winds_march *m) {
+    return (int64_t)m->g.nst * m->sc.target_ages * m->g.nb;
+}
+
+// feat rows [row0, row0+np) for the points P; network output rows likewise
+int mlp_features(winds_march *m, const double *X, int A, const double *gtp, double t,
+                 const double *P, int64_t np, int64_t row0) {
+    const float *nin = (const float *)m->norm.p;
+    WM_HIP(winds::surr::launch_features(geo_of(m, A, t), X, gtp, P, np, nin, (float *)m->desc.p,
+                                        (float *)m->feat.p + row0 * winds::surr::kNIn, m->st));
+    return WINDS_OK;
+}
+
+int mlp_apply(winds_march *m, int A, double t, int64_t np, int64_t row0, double *u3) {
+    const float *nout = (const float *)m->norm.p + 2 * winds::surr::kNIn;
+    WM_HIP(winds::surr::launch_apply(geo_of(m, A, t), (const float *)m->yout.p + row0 * 3, np,
+                                     nout, u3, m->st));
+    return WINDS_OK;
+}
+
+// u3 += the far wake's induced velocity at P by the configured method
+int far_part(winds_march *m, const double *X, int A, const double *gsp, const double *gtp,
+             const double *rsp, const double *rtp, double t, const double *P, int64_t np,
+             double *u3, const winds::EvalCfg &c, double *scratch) {
+    if (m->sc.mode == WINDS_SURR_DROP || np == 0) return WINDS_OK;
+    if (m->sc.mode == WINDS_SURR_MLP && !m->fallback) {
+        int rc = mlp_features(m, X, A, gtp, t, P, np, 0);
+        if (!rc) rc = winds_mlp_forward(m->net, (const float *)m->feat.p, (int32_t)np,
+                                        (float *)m->yout.p, m->st);
+        if (!rc) rc = mlp_apply(m, A, t, np, 0, u3);
+        return rc;
+    }
+    int rc = induced(m, X, A, gsp, gtp, rsp, rtp, P, np, scratch, c, 0, kFar);
+    if (rc) return rc;
+    k_accumulate<<<blocks(3 * np), 256, 0, m->st>>>(3 * np, scratch, u3);
+    WM_HIP(hipGetLastError());
+    return WINDS_OK;
+}
+
+// Direct far sum at the near-wake targets and the lifting line of time t,
+// the network's prediction at the same points before it is trained on them,
+// the fallback decision and the online training step.
+int surr_check(winds_march *m, const double *X, int A, const double *gsp, const double *gtp,
+               const double *rsp, const double *rtp, double t) {
+    const int St = m->sc.target_ages;
+    const int64_t Nt = near_targets(m);
+    const int NLL = m->g.ns * m->g.nb;
+    k_gather_near<<<blocks(Nt), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, St, X, m->tgt.d());
+    WM_HIP(hipGetLastError());
+    k_ll_points<<<blocks(NLL), 256, 0, m->st>>>(m->g, t, m->llp.d());
+    WM_HIP(hipGetLastError());
+    m->stats[0] += 1;
+    if (m->sc.mode != WINDS_SURR_MLP) return WINDS_OK;
+    const int64_t nf = pack_segments(m, X, A, gsp, gtp, rsp, rtp, 0, kFar);
+    int rc = eval_packed(m, nf, m->tgt.d(), Nt, m->ufar.d(), m->wake);
+    if (!rc) rc = eval_packed(m, nf, m->llp.d(), NLL, m->llfar.d(), m->ll);
+    if (!rc) rc = mlp_features(m, X, A, gtp, t, m->tgt.d(), Nt, 0);
+    if (!rc) rc = mlp_features(m, X, A, gtp, t, m->llp.d(), NLL, Nt);
+    const int32_t rows = (int32_t)(Nt + NLL);
+    if (!rc) rc = winds_mlp_forward(m->net, (const float *)m->feat.p, rows, (float *)m->yout.p,
+                                    m->st);
+    if (rc) return rc;
+    k_zero<<<blocks(3 * Nt), 256, 0, m->st>>>(3 * Nt, m->umlp.d());
+    k_zero<<<blocks(3 * NLL), 256, 0, m->st>>>(3 * NLL, m->llmlp.d());
+    WM_HIP(hipGetLastError());
+    rc = mlp_apply(m, A, t, Nt, 0, m->umlp.d());
+    if (!rc) rc = mlp_apply(m, A, t, NLL, Nt, m->llmlp.d());
+    if (rc) return rc;
+    unsigned long long *e = (unsigned long long *)m->err.p;
+    WM_HIP(hipMemsetAsync(e, 0, 4 * sizeof(unsigned long long), m->st));
+    k_max_err<<<blocks(Nt), 256, 0, m->st>>>(Nt, m->umlp.d(), m->ufar.d(), e);
+    k_max_err<<<blocks(NLL), 256, 0, m->st>>>(NLL, m->llmlp.d(), m->llfar.d(), e + 2);
+    WM_HIP(hipGetLastError());
+    double h[4];
+    WM_HIP(hipMemcpyAsync(h, e, sizeof(h), hipMemcpyDeviceToHost, m->st));
+    WM_HIP(hipStreamSynchronize(m->st));
+    const double en = h[1] > 0.0 ? h[0] / h[1] : 0.0;
+    const double el = h[3] > 0.0 ? h[2] / h[3] : 0.0;
+    m->stats[4] = en;
+    m->stats[5] = el;
+    m->stats[6] = std::max(m->stats[6], std::max(en, el));
+    m->fallback = !(en <= m->sc.tol && el <= m->sc.tol);
+    if (m->fallback) m->stats[1] += 1;
+    if (m->sc.train_iters > 0) {
+        float *lab = (float *)m->lab.p;
+        const float *nout = (const float *)m->norm.p + 2 * winds::surr::kNIn;
+        WM_HIP(winds::surr::launch_labels(geo_of(m, A, t), m->ufar.d(), Nt, nout, lab, m->st));
+        WM_HIP(winds::surr::launch_labels(geo_of(m, A, t), m->llfar.d(), NLL, nout, lab + 3 * Nt,
+                                          m->st));
+        for (int i = 0; i < m->sc.train_iters; ++i) {
+            rc = winds_mlp_train_step(m->net, (const float *)m->feat.p, lab, rows,
+                                      (float)m->sc.lr, nullptr, m->st);
+            if (rc) return rc;
+            m->stats[7] += 1;
+        }
+    }
+    return WINDS_OK;
+}
+
+enum How { kDirect = 0, kDirectCheck = 1, kSurrogate = 2 };
+
+// Induced velocity of the wake at its own nodes, into u (SoA [3][N]).
+int wake_velocity(winds_march *m, const double *X, int A, const double *gsp, const double *gtp,
+                  const double *rsp, const double *rtp, double t, double *u, How how) {
+    const int64_t N = nodes_of(m, A);
+    if (how != kSurrogate) {
+        int rc = induced(m, X, A, gsp, gtp, rsp, rtp, X, N, u, m->wake, 0);
+        if (rc || how == kDirect) return rc;
+        const double ps = psi_of(m, t);
+        k_store_turning<<<blocks(N), 256, 0, m->st>>>(N, cos(ps), sin(ps), u, m->ufz.d());
+        WM_HIP(hipGetLastError());
+        m->A_ref = A;
+        return surr_check(m, X, A, gsp, gtp, rsp, rtp, t);
+    }
+    const int St = m->sc.target_ages;
+    const int64_t Nt = near_targets(m);
+    k_gather_near<<<blocks(Nt), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, St, X, m->tgt.d());
+    WM_HIP(hipGetLastError());
+    int rc = induced(m, X, A, gsp, gtp, rsp, rtp, m->tgt.d(), Nt, m->unear.d(), m->wake, 0, kNear);
+    if (!rc)
+        rc = far_part(m, X, A, gsp, gtp, rsp, rtp, t, m->tgt.d(), Nt, m->unear.d(), m->wake,
+                      m->ufar.d());
+    if (rc) return rc;
+    const double ps = psi_of(m, t);
+    k_compose<<<blocks(N), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, St, m->unear.d(), m->ufz.d(),
+                                            m->A_ref, cos(ps), sin(ps), u);
+    WM_HIP(hipGetLastError());
+    return WINDS_OK;
+}
+
 // initials.m:44-152 (time index 1)
 int setup(winds_march *m) {
     const int A = 2;
@@ -787,7 +1069,11 @@ int step_once(winds_march *m) {
     rc = vcore(m, p, X, A, gs, gt, lprev, lnow, m->rs.d(), m->rt.d());
     if (rc) return rc;
     const int64_t N = nodes_of(m, A);
-    rc = induced(m, X, A, gs, gt, m->rs.d(), m->rt.d(), X, N, m->uind.d(), m->wake, 0);
+    // surrogate steps: a direct sum with a check every check_every steps
+    const bool sa = surr_on(m, A);
+    const bool chk = sa && (m->A_ref == 0 || m->since_check % m->sc.check_every == 0);
+    const How how = !sa ? kDirect : (chk ? kDirectCheck : kSurrogate);
+    rc = wake_velocity(m, X, A, gs, gt, m->rs.d(), m->rt.d(), t, m->uind.d(), how);
     if (rc) return rc;
     k_add_uinf<<<blocks(N), 256, 0, m->st>>>(N, m->cfg.uinf[0], m->cfg.uinf[1], m->cfg.uinf[2],
                                              m->uind.d(), m->vel.d());
@@ -808,8 +1094,9 @@ int step_once(winds_march *m) {
                            m->rt1.d());
             const int64_t Nn = nodes_of(m, A_new);
             if (!rc)
-                rc = induced(m, Xn, A_new, m->gs1.d(), m->gt1.d(), m->rs1.d(), m->rt1.d(), Xn, Nn,
-                             m->uind.d(), m->wake, 0);
+                rc = wake_velocity(m, Xn, A_new, m->gs1.d(), m->gt1.d(), m->rs1.d(), m->rt1.d(),
+                                   t + m->cfg.dt, m->uind.d(),
+                                   how == kSurrogate ? kSurrogate : kDirect);
             if (!rc) {
                 k_add_uinf<<<blocks(Nn), 256, 0, m->st>>>(Nn, m->cfg.uinf[0], m->cfg.uinf[1],
                                                           m->cfg.uinf[2], m->uind.d(), m->velp.d());
@@ -823,7 +1110,16 @@ int step_once(winds_march *m) {
     const int NLL = m->g.ns * m->g.nb;
     k_ll_points<<<blocks(NLL), 256, 0, m->st>>>(m->g, t, m->llp.d());
     WM_HIP(hipGetLastError());
-    rc = induced(m, X, A, gs, gt, m->rs.d(), m->rt.d(), m->llp.d(), NLL, m->frozen.d(), m->ll, 1);
+    if (how == kSurrogate) {
+        rc = induced(m, X, A, gs, gt, m->rs.d(), m->rt.d(), m->llp.d(), NLL, m->frozen.d(), m->ll,
+                     1, kNear);
+        if (!rc)
+            rc = far_part(m, X, A, gs, gt, m->rs.d(), m->rt.d(), t, m->llp.d(), NLL,
+                          m->frozen.d(), m->ll, m->llfar.d());
+    } else {
+        rc = induced(m, X, A, gs, gt, m->rs.d(), m->rt.d(), m->llp.d(), NLL, m->frozen.d(), m->ll,
+                     1);
+    }
     if (rc) return rc;
     const int NCH = (2 * m->g.ns + m->g.nst) * m->g.nb;
     k_influence<<<blocks((int64_t)NLL * NCH), 256, 0, m->st>>>(
@@ -839,6 +1135,11 @@ int step_once(winds_march *m) {
     WM_HIP(hipGetLastError());
     rc = push_hist(m, m->vel.d(), A);
     if (rc) return rc;
+    if (sa) {
+        if (how == kSurrogate)
+            m->stats[(m->sc.mode == WINDS_SURR_MLP && !m->fallback) ? 2 : 3] += 1;
+        m->since_check++;
+    }
     m->nsteps_done++;
     m->lcur = lnow;
     m->gprev = gc;
@@ -938,6 +1239,7 @@ int winds_march_create(const winds_march_cfg *cfg, winds_march **out) {
     m->g.uy = c.uinf[1];
     m->g.uz = c.uinf[2];
     const double umag = sqrt(c.uinf[0] * c.uinf[0] + c.uinf[1] * c.uinf[1] + c.uinf[2] * c.uinf[2]);
+    m->umag = umag;
     // initials.m:134-135
     const double T0 = 2 * kPi * c.R / (12 * c.tsr * umag);
     m->r0 = sqrt(4 * kAlphaRL * kNu * kDelta * T0);
@@ -1007,12 +1309,14 @@ int winds_march_sync(winds_march *m) {
     return WINDS_OK;
 }
 
-int winds_march_info(const winds_march *m, int64_t out[4]) {
+int winds_march_info(const winds_march *m, int64_t out[6]) {
     if (!m || !out) return set_error(WINDS_E_ARG, "null argument");
     out[0] = m->p;
     out[1] = m->A;
     out[2] = nodes_of(m, m->A);
     out[3] = segs_of(m, m->A);
+    out[4] = m->A_gprev;
+    out[5] = m->nsteps_done;
     return WINDS_OK;
 }
 
@@ -1082,6 +1386,8 @@ int winds_march_set_state(winds_march *m, int32_t p, int32_t ages, const double 
                                             m->rs.d(), m->rt.d());
     WM_HIP(hipGetLastError());
     m->ls_A[m->lcur] = A;
+    m->A_ref = 0;
+    m->since_check = 0;
     return WINDS_OK;
 }
 
@@ -1109,5 +1415,75 @@ int winds_march_get_kj(winds_march *m, double *cl, double *cd, double *aoa) {
 }
 
 const double *winds_march_uind_dev(const winds_march *m) { return m ? m->uind.d() : nullptr; }
+
+int winds_march_set_surrogate(winds_march *m, winds_mlp *net, const winds_march_surr *s) {
+    winds::clear_error();
+    if (!m) return set_error(WINDS_E_ARG, "null march");
+    WM_HIP(hipStreamSynchronize(m->st));
+    if (!s) {
+        m->surr = false;
+        m->net = nullptr;
+        return WINDS_OK;
+    }
+    const winds_march_surr &c = *s;
+    if (c.mode < WINDS_SURR_MLP || c.mode > WINDS_SURR_DROP)
+        return set_error(WINDS_E_MODE, "unknown surrogate mode");
+    if (c.split < 2 || c.split >= m->cfg.max_ages || c.target_ages < 1 || c.target_ages > c.split ||
+        c.check_every < 1 || c.train_iters < 0)
+        return set_error(WINDS_E_DIMS, "bad surrogate split/targets/check interval");
+    if (!(c.tol > 0.0) || !(c.lr >= 0.0)) return set_error(WINDS_E_ARG, "bad tol/lr");
+    const int nst = m->g.nst, nb = m->g.nb, NLL = m->g.ns * m->g.nb;
+    const int64_t Nt = (int64_t)nst * c.target_ages * nb;
+    const int64_t rows = Nt + NLL;
+    constexpr int NIN = winds::surr::kNIn;
+    if (c.mode == WINDS_SURR_MLP) {
+        if (!net || !c.in_norm || !c.out_norm)
+            return set_error(WINDS_E_ARG, "network mode needs a network and normalisation");
+        int64_t info[6];
+        int rc = winds_mlp_info(net, info);
+        if (rc) return rc;
+        if (info[0] != NIN || info[3] != winds::surr::kNOut)
+            return set_error(WINDS_E_DIMS, "network inputs/outputs do not match the surrogate");
+        if (info[4] < rows)
+            return set_error(WINDS_E_DIMS, "network max_batch below near targets + lifting line");
+    }
+    const size_t D = sizeof(double);
+    const size_t NB = (size_t)3 * nst * m->cfg.max_ages * nb * D;
+    WM_HIP(m->norm.alloc((2 * NIN + 6) * sizeof(float)));
+    WM_HIP(m->tgt.alloc(3 * Nt * D));
+    WM_HIP(m->unear.alloc(3 * Nt * D));
+    WM_HIP(m->ufar.alloc(3 * Nt * D));
+    WM_HIP(m->umlp.alloc(3 * Nt * D));
+    WM_HIP(m->ufz.alloc(NB));
+    WM_HIP(m->llfar.alloc(3 * NLL * D));
+    WM_HIP(m->llmlp.alloc(3 * NLL * D));
+    WM_HIP(m->feat.alloc(rows * NIN * sizeof(float)));
+    WM_HIP(m->lab.alloc(rows * 3 * sizeof(float)));
+    WM_HIP(m->yout.alloc(rows * 3 * sizeof(float)));
+    WM_HIP(m->desc.alloc(64 * sizeof(float)));
+    WM_HIP(m->err.alloc(4 * sizeof(unsigned long long)));
+    if (c.mode == WINDS_SURR_MLP) {
+        std::vector<float> h(2 * NIN + 6);
+        std::copy(c.in_norm, c.in_norm + 2 * NIN, h.begin());
+        std::copy(c.out_norm, c.out_norm + 6, h.begin() + 2 * NIN);
+        WM_HIP(hipMemcpy(m->norm.p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    m->net = c.mode == WINDS_SURR_MLP ? net : nullptr;
+    m->sc = c;
+    m->sc.in_norm = nullptr;  // copied to the device
+    m->sc.out_norm = nullptr;
+    m->surr = true;
+    m->fallback = true;
+    m->A_ref = 0;
+    m->since_check = 0;
+    for (double &v : m->stats) v = 0.0;
+    return WINDS_OK;
+}
+
+int winds_march_surr_stats(const winds_march *m, double out[8]) {
+    if (!m || !out) return set_error(WINDS_E_ARG, "null argument");
+    for (int i = 0; i < 8; ++i) out[i] = m->stats[i];
+    return WINDS_OK;
+}
 
 }  // extern "C"

@@ -403,6 +403,17 @@ int64_t winds_mlp_num_params(const winds_mlp *m) {
     return n;
 }
 
+int winds_mlp_info(const winds_mlp *m, int64_t out[6]) {
+    if (!m || !out) return set_error(WINDS_E_ARG, "null argument");
+    out[0] = m->n_in;
+    out[1] = m->hidden;
+    out[2] = m->nl - 1;
+    out[3] = m->n_out;
+    out[4] = m->Bmax;
+    out[5] = winds_mlp_num_params(m);
+    return WINDS_OK;
+}
+
 int winds_mlp_get_params(winds_mlp *m, float *host) {
     winds::clear_error();
     if (!m || !host) return set_error(WINDS_E_ARG, "null argument");
@@ -442,7 +453,7 @@ int winds_mlp_forward(winds_mlp *m, const float *X, int32_t B, float *Y, void *s
     if (B < 0 || B > m->Bmax) return set_error(WINDS_E_DIMS, "batch larger than max_batch");
     if (B == 0) return WINDS_OK;
     if (!X || !Y) return set_error(WINDS_E_ARG, "null input/output");
-    hipStream_t st = stream ? (hipStream_t)stream : m->st;
+    hipStream_t st = (hipStream_t)stream;  // NULL: the default stream
     const int Bp = rup(B, 64);
     int rc = forward(m, X, B, Bp, false, st);
     if (rc) return rc;
@@ -457,7 +468,7 @@ int winds_mlp_train_step(winds_mlp *m, const float *X, const float *T, int32_t B
     winds::clear_error();
     if (!m || !X || !T) return set_error(WINDS_E_ARG, "null argument");
     if (B < 1 || B > m->Bmax) return set_error(WINDS_E_DIMS, "batch out of range");
-    hipStream_t st = stream ? (hipStream_t)stream : m->st;
+    hipStream_t st = (hipStream_t)stream;  // NULL: the default stream
     const int Bp = rup(B, 64);
     if (loss) MLP_HIP(hipMemsetAsync(loss, 0, sizeof(float), st));
     int rc = forward(m, X, B, Bp, true, st);

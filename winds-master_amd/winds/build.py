@@ -25,6 +25,7 @@ SOURCES = [
     "bs_batch.hip",
     "march.hip",
     "mlp.hip",
+    "surrogate.hip",
 ]
 
 ARCH = "gfx950"
@@ -85,7 +86,29 @@ def build(verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + _quiet(r.stderr))
     os.replace(tmp, LIB)
+    build_mex_mock(verbose)
     return LIB
+
+
+MEX_DIR = PKG_ROOT / "mex"
+MEX_MOCK_LIB = LIBDIR / "libwinds_mexmock.so"
+
+
+def build_mex_mock(verbose: bool = False) -> Path:
+    """The MEX gateway (mex/BiotSavartMex.cpp) linked against the in-process
+    mock of MATLAB's MEX runtime (mex/mock), for tests: host code only, g++."""
+    tmp = MEX_MOCK_LIB.with_suffix(".so.tmp")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
+           f"-I{MEX_DIR / 'mock'}", f"-I{REPO_ROOT / 'include'}",
+           str(MEX_DIR / "BiotSavartMex.cpp"), str(MEX_DIR / "mock" / "mock_mex.cpp"),
+           "-o", str(tmp), f"-L{LIBDIR}", "-lwinds_bs", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("mock MEX build failed:\n" + r.stderr)
+    os.replace(tmp, MEX_MOCK_LIB)
+    return MEX_MOCK_LIB
 
 
 if __name__ == "__main__":

@@ -145,10 +145,11 @@ def rotor_wake(nb: int = 3, nst: int = 18, ages: int = 72, R: float = 63.0,
                gamma_peak: float = 60.0, gamma_noise: float = 0.05,
                r0: float = 0.05, alpha: float = 1.25613, delta: float = 100.0,
                nu: float = 1e-5, yaw_deg: float = 0.0, seed_pos: int = 1,
-               seed_gamma: int = 2) -> RotorWake:
+               seed_gamma: int = 2, psi0: float = 0.0) -> RotorWake:
     """Helical wake of a rigid rotor with seeded jitter and circulation noise.
 
-    Node (j, k, b) sits at station radius r_j, azimuth psi_b - k*Omega*dt and
+    Node (j, k, b) sits at station radius r_j, azimuth
+    psi_b - k*Omega*dt with psi_b = psi0 + 2 pi b / nb, and
     axial distance U(1-a)*k*dt along the convection direction, which is the
     +x axis yawed by yaw_deg about z.  Every coordinate gets N(0, jitter*R).
     Core: r_c^2 = r0^2 + 4*alpha*delta*nu*t_age, with t_age = k*dt for shed
@@ -158,7 +159,7 @@ def rotor_wake(nb: int = 3, nst: int = 18, ages: int = 72, R: float = 63.0,
     dt = math.radians(dpsi_deg) / omega
     r = cosine_stations(nst, R, hub)
     k = np.arange(ages, dtype=np.float64)
-    psi_b = 2.0 * math.pi * np.arange(nb) / nb
+    psi_b = psi0 + 2.0 * math.pi * np.arange(nb) / nb
     psi = psi_b[None, :] - (k * omega * dt)[:, None]            # [ages, nb]
     s = U * (1.0 - a) * k * dt                                   # [ages]
     yaw = math.radians(yaw_deg)
@@ -189,7 +190,7 @@ def rotor_wake(nb: int = 3, nst: int = 18, ages: int = 72, R: float = 63.0,
                      dt=dt, omega=omega,
                      params=dict(nb=nb, nst=nst, ages=ages, R=R, hub=hub, tsr=tsr, U=U,
                                  a=a, yaw_deg=yaw_deg, seed_pos=seed_pos,
-                                 seed_gamma=seed_gamma))
+                                 seed_gamma=seed_gamma, psi0=psi0))
 
 
 def config1() -> RotorWake:

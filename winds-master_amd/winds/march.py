@@ -23,6 +23,19 @@ class MarchCfg(C.Structure):
     ]
 
 
+class MarchSurr(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32), ("split", C.c_int32), ("target_ages", C.c_int32),
+        ("check_every", C.c_int32), ("train_iters", C.c_int32), ("reserved", C.c_int32),
+        ("tol", C.c_double), ("lr", C.c_double), ("in_norm", C.c_void_p),
+        ("out_norm", C.c_void_p),
+    ]
+
+
+SURR_STATS = ("checks", "failed_checks", "network_steps", "direct_far_steps", "last_err_nodes",
+              "last_err_ll", "max_err", "adam_steps")
+
+
 def _bind(L):
     if getattr(L, "_winds_march_bound", False):
         return L
@@ -40,7 +53,10 @@ def _bind(L):
     L.winds_march_get_kj.argtypes = [P, D, D, D]
     L.winds_march_uind_dev.argtypes = [P]
     L.winds_march_uind_dev.restype = C.c_void_p
-    for f in ("create", "step", "sync", "info", "get_state", "set_state", "get_perf", "get_kj"):
+    L.winds_march_set_surrogate.argtypes = [P, P, C.POINTER(MarchSurr)]
+    L.winds_march_surr_stats.argtypes = [P, D]
+    for f in ("create", "step", "sync", "info", "get_state", "set_state", "get_perf", "get_kj",
+              "set_surrogate", "surr_stats"):
         getattr(L, "winds_march_" + f).restype = C.c_int
     L._winds_march_bound = True
     return L
@@ -86,8 +102,11 @@ class March:
     def __del__(self):
         m = getattr(self, "_m", None)
         if m is not None and m.value:
-            self.L.winds_march_destroy(m)
-            self._m = C.c_void_p()
+            try:
+                self.L.winds_march_destroy(m)
+                m.value = None
+            except Exception:       # interpreter shutdown
+                pass
 
     def step(self, n: int = 1):
         _lib.check(self.L.winds_march_step(self._m, int(n)))
@@ -96,9 +115,10 @@ class March:
         _lib.check(self.L.winds_march_sync(self._m))
 
     def info(self) -> dict:
-        a = (C.c_int64 * 4)()
+        a = (C.c_int64 * 6)()
         _lib.check(self.L.winds_march_info(self._m, a))
-        return {"p": a[0], "ages": a[1], "nodes": a[2], "segments": a[3]}
+        return {"p": a[0], "ages": a[1], "nodes": a[2], "segments": a[3], "ages_solved": a[4],
+                "steps": a[5]}
 
     def state(self):
         """(nodes [nst,3,A,nb] of the next time index, shed [ns,1,A',nb] and
@@ -130,7 +150,7 @@ class March:
 
     def perf(self, first: int = 0, n: int | None = None) -> np.ndarray:
         """Rows (p, T, Q, P, KJ iterations) of the stepped time indices."""
-        done = self.info()["p"] - 2
+        done = self.info()["steps"]
         if n is None:
             n = done - first
         out = np.zeros((n, 5))
@@ -143,3 +163,29 @@ class March:
         _lib.check(self.L.winds_march_get_kj(self._m, _dp(cl), _dp(cd), _dp(aoa)))
         f = lambda a: a.reshape((r.ns, r.nb), order="F")
         return f(cl), f(cd), f(aoa)
+
+    def set_surrogate(self, cfg=None, net=None, in_norm=None, out_norm=None):
+        """Turn the far-wake surrogate on (cfg: winds.surrogate.MarchSurrogate)
+        or off (cfg None).  The network must outlive its use by this march."""
+        if cfg is None:
+            _lib.check(self.L.winds_march_set_surrogate(self._m, None, None))
+            self._surr_keep = None
+            return
+        s = MarchSurr()
+        s.mode, s.split, s.target_ages = cfg.mode, cfg.split, cfg.target_ages
+        s.check_every, s.train_iters = cfg.check_every, cfg.train_iters
+        s.tol, s.lr = cfg.tol, cfg.lr
+        keep = []
+        if in_norm is not None:
+            a = np.ascontiguousarray(in_norm, dtype=np.float32)
+            b = np.ascontiguousarray(out_norm, dtype=np.float32)
+            keep = [a, b]
+            s.in_norm, s.out_norm = a.ctypes.data, b.ctypes.data
+        self._surr_keep = (net, keep)
+        _lib.check(self.L.winds_march_set_surrogate(self._m, net.handle if net is not None else None,
+                                                    C.byref(s)))
+
+    def surr_stats(self) -> dict:
+        a = np.zeros(8)
+        _lib.check(self.L.winds_march_surr_stats(self._m, _dp(a)))
+        return dict(zip(SURR_STATS, a.tolist()))
