@@ -164,27 +164,30 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
     from winds.mlp import MLP
 
     o = march_opts(winds.PREC_FP32C)
-    ts = S.build_training_set(n_samples, o, seed=7, log=log)
+    sc = S.MarchSurrogate()
+    ts = S.build_training_set(n_samples, o, seed=7, split=sc.split, target_ages=sc.target_ages,
+                              log=log)
     log(f"surrogate data: {ts.wakes} synthetic wakes, {ts.F.shape[0]} samples, "
         f"{ts.interactions / 1e9:.1f} G interactions, {ts.seconds:.1f} s")
-    held = S.build_training_set(50_000, o, seed=8)
-    sets = [ts]
-    if march_set is not None:
-        # the march's own samples, repeated to weigh about as much as the synthetic ones
-        rep = max(1, int(round(0.5 * ts.F.shape[0] / max(1, march_set.F.shape[0]))))
-        sets += [march_set] * rep
-    full = S.concat(*sets)
-    in_norm, out_norm = S.fit_norm(full.F, full.Y)
-    Fn, Yn = S.normalise(full.F, full.Y, in_norm, out_norm)
-    Hn, _ = S.normalise(held.F, None, in_norm, out_norm)
-    rows = 31 * 108 * 3 + 30 * 3
+    held = S.build_training_set(50_000, o, seed=8, split=sc.split, target_ages=sc.target_ages)
+    rows = 31 * sc.target_ages * 3 + 30 * 3
     net = MLP(S.NIN, MLP_HIDDEN, MLP_LAYERS, S.NOUT, max(batch, rows), seed=7)
-    secs, loss = S.train(net, Fn, Yn, steps=train_steps, batch=batch, lr=1e-3, log=log)
+    if march_set is not None:
+        in_norm, out_norm, (s1, s2) = S.train_two_phase(net, ts, march_set, train_steps, batch,
+                                                        log=log)
+        secs = s1 + s2
+        loss = None
+    else:
+        in_norm, out_norm = S.fit_norm(ts.F, ts.Y)
+        Fn, Yn = S.normalise(ts.F, ts.Y, in_norm, out_norm)
+        secs, loss = S.train(net, Fn, Yn, steps=train_steps, batch=batch, lr=1e-3, log=log)
+        del Fn, Yn
+    Hn, _ = S.normalise(held.F, None, in_norm, out_norm)
     W = mlp_weights(S.NIN, MLP_HIDDEN, MLP_LAYERS, S.NOUT)
     # training: forward 2W, weight gradients 2W, input gradients 2W (less the first layer)
     train_flop = (6 * W - 2 * S.NIN * MLP_HIDDEN) * batch
     train_tflops = train_flop * train_steps / secs / 1e12
-    Ft, _ = S.normalise(ts.F[:200_000], None, in_norm, out_norm)
+    Ft, _ = S.normalise(ts.F[:200_000].contiguous(), None, in_norm, out_norm)
     err_train = S.normwise_error(S.predict(net, Ft, out_norm), ts.Y[:200_000])
     err_held = S.normwise_error(S.predict(net, Hn, out_norm), held.Y)
     err_march = None
@@ -192,11 +195,11 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
         Fm, _ = S.normalise(march_set.F, None, in_norm, out_norm)
         err_march = S.normwise_error(S.predict(net, Fm, out_norm), march_set.Y)
     log(f"surrogate training: {train_steps} steps x {batch}: {secs:.1f} s "
-        f"({train_tflops:.1f} TFLOP/s), loss {loss:.3e}, normwise error synthetic train "
+        f"({train_tflops:.1f} TFLOP/s), loss {loss}, normwise error synthetic train "
         f"{err_train:.3e} held-out {err_held:.3e} march samples {err_march}")
     # inference rate on max_batch rows
     B = net.max_batch
-    Xb = Fn[:B].contiguous()
+    Xb = Ft[:B].contiguous()
     Yb = torch.empty((B, S.NOUT), dtype=torch.float32, device=Xb.device)
     for _ in range(3):
         net.forward(Xb, Yb)
@@ -217,7 +220,8 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
         "network": f"{S.NIN} inputs -> {MLP_LAYERS} x {MLP_HIDDEN} SiLU -> 3, BF16 weights and "
                    f"activations, FP32 accumulation (v_mfma_f32_32x32x8bf16_1k)",
         "parameters": net.num_params(),
-        "split_age": 144, "target_ages": 108,
+        "split_age": sc.split, "target_ages": sc.target_ages,
+        "training": "half the steps on synthetic wakes, half on the march's own samples",
         "synthetic_samples": int(ts.F.shape[0]), "synthetic_wakes": ts.wakes,
         "synthetic_data_G_interactions": round(ts.interactions / 1e9, 2),
         "synthetic_data_s": round(ts.seconds, 2),
@@ -272,7 +276,7 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
     log(f"march cold start: {first} steps in {time.perf_counter() - t0:.1f} s")
     msamp = None
     if surrogate:
-        msamp = S.march_training_set(m, o32, steps=dev_steps - first, every=24,
+        msamp = S.march_training_set(m, o32, steps=dev_steps - first, every=12,
                                      split=sc.split, target_ages=sc.target_ages, log=log)
     else:
         m.step(dev_steps - first)
@@ -408,7 +412,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     dev = torch.device("cuda", 0)
     wake = gen.rotor_wake(nb=3, nst=31, ages=args.ages)
-    recs = layout.wake_segments(wake)
+    # trailing filaments line by line: the engine's own order (march.hip)
+    recs = layout.wake_segments(wake, chained=True)
     nt, nseg = wake.n_targets, recs.shape[0]
     inter = nt * nseg
     log(f"workload: {nt} targets x {nseg} segments = {inter / 1e9:.3f} G interactions")

@@ -116,10 +116,11 @@ const double *winds_march_uind_dev(const winds_march *m);
  * Every check_every steps (and on the first such step) the step is a full
  * direct sum; it refreshes the held velocities and, in WINDS_SURR_MLP,
  * compares the network with the direct far sum at the near targets and the
- * lifting line (normwise: max |u_net - u_far| / max |u_far|, each point set
- * separately).  A check above tol switches the following steps to the
- * direct far sum until a check passes; train_iters Adam steps (rate lr) on
- * the check's direct sums follow every check.  The network needs
+ * lifting line: error = max |u_net - u_far| / max |u_ind| over each point
+ * set, u_ind being the whole wake's induced velocity there (the velocity the
+ * network's error perturbs).  A check above tol switches the following steps
+ * to the direct far sum until a check passes; train_iters Adam steps (rate
+ * lr) on the check's direct sums follow every check.  The network needs
  * WINDS_SURR_NIN inputs, 3 outputs and max_batch >= nst*target_ages*nb +
  * (nst-1)*nb; it is not owned and must outlive its use.  s == NULL turns the
  * surrogate off. */
@@ -143,8 +144,9 @@ typedef struct winds_march_surr {
 int winds_march_set_surrogate(winds_march *m, winds_mlp *net, const winds_march_surr *s);
 
 /* out = {checks, failed checks, network steps, direct-far steps, last node
- * error, last lifting-line error, largest error, Adam steps}. */
-int winds_march_surr_stats(const winds_march *m, double out[8]);
+ * error, last lifting-line error, largest error, Adam steps, last node and
+ * lifting-line errors relative to max |u_far| instead of max |u_ind|}. */
+int winds_march_surr_stats(const winds_march *m, double out[10]);
 
 #ifdef __cplusplus
 }

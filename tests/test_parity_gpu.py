@@ -88,11 +88,12 @@ def test_fp32c_config1(c1, cmod, d):
         assert s["max_rel"] <= 5e-6 and s["rms_rel"] <= 3e-7 and s["normwise"] <= 1e-7
 
 
+@pytest.mark.parametrize("chained", [False, True])
 @pytest.mark.parametrize("core", ["vatistas2", "vatistas1", "cutoff"])
-def test_fp32c_config1_whole_wake(c1, core):
+def test_fp32c_config1_whole_wake(c1, core, chained):
     import torch
     from winds.device import Evaluator
-    recs = layout.wake_segments(c1)
+    recs = layout.wake_segments(c1, chained=chained)
     tg = torch.from_numpy(layout.wake_targets(c1)).cuda().contiguous()
     sg = torch.from_numpy(recs).cuda().contiguous()
     F1 = np.asfortranarray(recs[:, 0:3].reshape(-1, 3, 1, 1))
@@ -108,7 +109,7 @@ def test_fp32c_config1_whole_wake(c1, core):
         ref = O.bs_matlab(F1, F2, c1.domain, recs[:, 6], recs[:, 7], f"visc{n}")
     u = layout.uind_to_cm(Evaluator(0).eval(tg, sg, o).cpu().numpy(), c1.domain.shape)
     s = stats(u, ref)
-    flush_print(f"[fp32c whole wake {core}] {s}")
+    flush_print(f"[fp32c whole wake {core} chained={chained}] {s}")
     assert s["max_rel"] <= 1e-6 and s["rms_rel"] <= 1e-7 and s["normwise"] <= 1e-7
 
 
@@ -236,7 +237,7 @@ def test_device_api_matches_host_api(c1):
 def c2_device(need_gpu):
     import torch
     w = gen.config2()
-    recs = layout.wake_segments(w)
+    recs = layout.wake_segments(w, chained=True)      # the order bench.py times
     tg = torch.from_numpy(layout.wake_targets(w)).cuda().contiguous()
     sg = torch.from_numpy(recs).cuda().contiguous()
     return w, recs, tg, sg

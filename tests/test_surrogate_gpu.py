@@ -106,8 +106,8 @@ def test_direct_far_with_held_far_velocities_within_one_percent(need_gpu):
     rot = small_rotor()
     steps = 36 * 10
     a, b, c = march(rot, 200), march(rot, 200), march(rot, 200)
-    b.set_surrogate(MarchSurrogate(mode=1, split=72, target_ages=54, check_every=18))
-    c.set_surrogate(MarchSurrogate(mode=2, split=72, target_ages=54, check_every=18))
+    b.set_surrogate(MarchSurrogate(mode=1, split=72, target_ages=36, check_every=18))
+    c.set_surrogate(MarchSurrogate(mode=2, split=72, target_ages=36, check_every=18))
     for m in (a, b, c):
         m.step(steps)
         m.sync()
@@ -137,7 +137,7 @@ def developed():
                      d=1e-4)
     m = march(rot, 200)
     m.step(108)
-    ts = S.march_training_set(m, o32, steps=108, every=3, split=72, target_ages=54)
+    ts = S.march_training_set(m, o32, steps=108, every=3, split=72, target_ages=36)
     inf = m.info()
     nodes, _, _ = m.state()
     A, Ag = inf["ages"], inf["ages_solved"]
@@ -156,18 +156,14 @@ def small_net(developed):
     _, _, tm = developed
     o = winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, precision=winds.PREC_FP32C,
                    d=1e-4)
-    tg = S.build_training_set(100_000, o, seed=7, split=72, target_ages=54, max_ages=200, nb=2,
+    tg = S.build_training_set(100_000, o, seed=7, split=72, target_ages=36, max_ages=200, nb=2,
                               R=20.0, U=10.0, hub=1.0, nst_range=(7, 7), dpsi_deg=10.0)
-    ts = S.concat(tg, *([tm] * 5))
-    in_norm, out_norm = S.fit_norm(ts.F, ts.Y)
-    Fn, Yn = S.normalise(ts.F, ts.Y, in_norm, out_norm)
     net = MLP(S.NIN, 256, 4, S.NOUT, 2048, seed=7)
-    secs, loss = S.train(net, Fn, Yn, steps=8000, batch=1024, lr=1e-3)
+    in_norm, out_norm, secs = S.train_two_phase(net, tg, tm, steps=8000, batch=1024)
     Fm, _ = S.normalise(tm.F, None, in_norm, out_norm)
     err = S.normwise_error(S.predict(net, Fm, out_norm), tm.Y)
     flush_print(f"[surrogate pretrain] {tg.wakes} synthetic wakes + {tm.wakes} march snapshots, "
-                f"{Fn.shape[0]} samples, {secs:.1f} s, loss {loss:.3e}, normwise error on the "
-                f"march samples {err:.3e}")
+                f"{sum(secs):.1f} s, normwise error on the march samples {err:.3e}")
     return net, in_norm, out_norm, err
 
 
@@ -182,9 +178,9 @@ def test_network_gains_over_dropping_the_far_wake(need_gpu, developed, small_net
     steps = 36 * 10
     a, b, c = (fork(state, rot, 200) for _ in range(3))
     # tol far above any error: the network is used on every surrogate step
-    b.set_surrogate(MarchSurrogate(mode=0, split=72, target_ages=54, check_every=18,
+    b.set_surrogate(MarchSurrogate(mode=0, split=72, target_ages=36, check_every=18,
                                    train_iters=20, tol=1e9, lr=1e-4), net, in_norm, out_norm)
-    c.set_surrogate(MarchSurrogate(mode=2, split=72, target_ages=54, check_every=18))
+    c.set_surrogate(MarchSurrogate(mode=2, split=72, target_ages=36, check_every=18))
     for m in (a, b, c):
         m.step(steps)
         m.sync()
@@ -193,7 +189,29 @@ def test_network_gains_over_dropping_the_far_wake(need_gpu, developed, small_net
     st = b.surr_stats()
     flush_print(f"[surrogate network] T/Q/P err {eb:.2e}; far wake dropped {ec:.2e}; {st}")
     assert st["network_steps"] > 0.8 * steps and st["adam_steps"] > 0
-    assert eb < 0.5 * ec
+    assert eb < 0.3 * ec
+
+
+def test_surrogate_march_within_one_percent(need_gpu, developed, small_net):
+    """The surrogate with checks, fallback and online training: thrust,
+    torque and power within 1% of the plain march.  The far wake matters more
+    to this small rotor than to the 1,440-age one, so the check tolerance is
+    0.5% of the induced velocity here."""
+    from winds.surrogate import MarchSurrogate
+    net, in_norm, out_norm, _ = small_net
+    rot, state, _ = developed
+    steps = 36 * 10
+    a, b = fork(state, rot, 200), fork(state, rot, 200)
+    b.set_surrogate(MarchSurrogate(mode=0, split=72, target_ages=36, check_every=18,
+                                   train_iters=20, tol=0.005, lr=1e-4), net, in_norm, out_norm)
+    for m in (a, b):
+        m.step(steps)
+        m.sync()
+    eb = rel_perf_err(b.perf(), a.perf())
+    st = b.surr_stats()
+    flush_print(f"[surrogate tol 0.5%] T/Q/P err {eb:.2e}; {st}")
+    assert st["checks"] == 20 and st["network_steps"] + st["direct_far_steps"] == steps - 20
+    assert st["network_steps"] > 0
     assert eb <= 0.01
 
 
@@ -203,7 +221,7 @@ def test_failed_check_falls_back_to_the_direct_sum(need_gpu, developed, small_ne
     rot, state, _ = developed
     m = fork(state, rot, 200)
     # a tolerance no network meets: every check fails, every step is direct
-    m.set_surrogate(MarchSurrogate(mode=0, split=72, target_ages=54, check_every=18,
+    m.set_surrogate(MarchSurrogate(mode=0, split=72, target_ages=36, check_every=18,
                                    train_iters=0, tol=1e-12), net, in_norm, out_norm)
     m.step(80)
     m.sync()

@@ -255,8 +255,9 @@ __global__ void k_vcore(int nst, int nb, int A, const double *__restrict__ X,
     }
 }
 
-// Segment records from nodes and circulation (shed first, then trailing),
-// for shed ages [s0, s1) and trailing ages [t0, t1) of every blade.
+// Segment records from nodes and circulation (shed first, in the reference's
+// order; then trailing, line by line), for shed ages [s0, s1) and trailing
+// ages [t0, t1) of every blade.
 // zero_changing zeroes the circulation of the filaments the Kutta-Joukowski
 // iteration changes: shed ages 0 and 1, trailing age 0.
 __global__ void k_segments(int nst, int nb, int A, const double *__restrict__ X,
@@ -283,10 +284,13 @@ __global__ void k_segments(int nst, int nb, int A, const double *__restrict__ X,
         s.g = (zero_changing && a <= 1) ? 0.0 : gs[k];
         s.rc = rs[k];
     } else {
+        // line by line (blade, station, age from the oldest down):
+        // consecutive records are neighbouring filaments
         const int64_t q = i - n_s;
-        const int m = (int)(q % nst);
-        const int a = t0 + (int)((q / nst) % wt);
         const int b = (int)(q / ((int64_t)nst * wt));
+        const int64_t r = q % ((int64_t)nst * wt);
+        const int m = (int)(r / wt);
+        const int a = t1 - 1 - (int)(r % wt);
         const int64_t k = m + (int64_t)a * nst + (int64_t)b * nst * (A - 1);
         j2 = m + (int64_t)a * nst + (int64_t)b * nst * A;
         j1 = j2 + nst;
@@ -739,7 +743,7 @@ struct winds_march {
     int A_ref = 0;         // ages of the turning-frame store ufz (0: empty)
     int64_t since_check = 0;
     DBuf norm, tgt, unear, ufar, umlp, ufz, llfar, llmlp, feat, lab, yout, desc, err;
-    double stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double stats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -928,9 +932,10 @@ int far_part(winds_march *m, const double *X, int A, const double *gsp, const do
 
 // Direct far sum at the near-wake targets and the lifting line of time t,
 // the network's prediction at the same points before it is trained on them,
-// the fallback decision and the online training step.
+// the fallback decision and the online training step.  u is the direct
+// induced velocity of the whole wake at its nodes (this check's step).
 int surr_check(winds_march *m, const double *X, int A, const double *gsp, const double *gtp,
-               const double *rsp, const double *rtp, double t) {
+               const double *rsp, const double *rtp, double t, const double *u) {
     const int St = m->sc.target_ages;
     const int64_t Nt = near_targets(m);
     const int NLL = m->g.ns * m->g.nb;
@@ -955,19 +960,30 @@ int surr_check(winds_march *m, const double *X, int A, const double *gsp, const 
     rc = mlp_apply(m, A, t, Nt, 0, m->umlp.d());
     if (!rc) rc = mlp_apply(m, A, t, NLL, Nt, m->llmlp.d());
     if (rc) return rc;
-    unsigned long long *e = (unsigned long long *)m->err.p;
-    WM_HIP(hipMemsetAsync(e, 0, 4 * sizeof(unsigned long long), m->st));
-    k_max_err<<<blocks(Nt), 256, 0, m->st>>>(Nt, m->umlp.d(), m->ufar.d(), e);
-    k_max_err<<<blocks(NLL), 256, 0, m->st>>>(NLL, m->llmlp.d(), m->llfar.d(), e + 2);
+    // the whole wake's induced velocity at the same points: the near
+    // targets' share of u, and a direct sum at the lifting line
+    k_gather_near<<<blocks(Nt), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, St, u, m->unear.d());
     WM_HIP(hipGetLastError());
-    double h[4];
+    rc = induced(m, X, A, gsp, gtp, rsp, rtp, m->llp.d(), NLL, m->frozen.d(), m->ll, 0);
+    if (rc) return rc;
+    // e = {max|net - far|, max|far|, -, max|u|} for the nodes, then the lifting line
+    unsigned long long *e = (unsigned long long *)m->err.p;
+    WM_HIP(hipMemsetAsync(e, 0, 8 * sizeof(unsigned long long), m->st));
+    k_max_err<<<blocks(Nt), 256, 0, m->st>>>(Nt, m->umlp.d(), m->ufar.d(), e);
+    k_max_err<<<blocks(Nt), 256, 0, m->st>>>(Nt, m->unear.d(), m->unear.d(), e + 2);
+    k_max_err<<<blocks(NLL), 256, 0, m->st>>>(NLL, m->llmlp.d(), m->llfar.d(), e + 4);
+    k_max_err<<<blocks(NLL), 256, 0, m->st>>>(NLL, m->frozen.d(), m->frozen.d(), e + 6);
+    WM_HIP(hipGetLastError());
+    double h[8];
     WM_HIP(hipMemcpyAsync(h, e, sizeof(h), hipMemcpyDeviceToHost, m->st));
     WM_HIP(hipStreamSynchronize(m->st));
-    const double en = h[1] > 0.0 ? h[0] / h[1] : 0.0;
-    const double el = h[3] > 0.0 ? h[2] / h[3] : 0.0;
+    auto ratio = [](double a, double b) { return b > 0.0 ? a / b : 0.0; };
+    const double en = ratio(h[0], h[3]), el = ratio(h[4], h[7]);
     m->stats[4] = en;
     m->stats[5] = el;
     m->stats[6] = std::max(m->stats[6], std::max(en, el));
+    m->stats[8] = ratio(h[0], h[1]);
+    m->stats[9] = ratio(h[4], h[5]);
     m->fallback = !(en <= m->sc.tol && el <= m->sc.tol);
     if (m->fallback) m->stats[1] += 1;
     if (m->sc.train_iters > 0) {
@@ -999,7 +1015,7 @@ int wake_velocity(winds_march *m, const double *X, int A, const double *gsp, con
         k_store_turning<<<blocks(N), 256, 0, m->st>>>(N, cos(ps), sin(ps), u, m->ufz.d());
         WM_HIP(hipGetLastError());
         m->A_ref = A;
-        return surr_check(m, X, A, gsp, gtp, rsp, rtp, t);
+        return surr_check(m, X, A, gsp, gtp, rsp, rtp, t, u);
     }
     const int St = m->sc.target_ages;
     const int64_t Nt = near_targets(m);
@@ -1461,7 +1477,7 @@ int winds_march_set_surrogate(winds_march *m, winds_mlp *net, const winds_march_
     WM_HIP(m->lab.alloc(rows * 3 * sizeof(float)));
     WM_HIP(m->yout.alloc(rows * 3 * sizeof(float)));
     WM_HIP(m->desc.alloc(64 * sizeof(float)));
-    WM_HIP(m->err.alloc(4 * sizeof(unsigned long long)));
+    WM_HIP(m->err.alloc(8 * sizeof(unsigned long long)));
     if (c.mode == WINDS_SURR_MLP) {
         std::vector<float> h(2 * NIN + 6);
         std::copy(c.in_norm, c.in_norm + 2 * NIN, h.begin());
@@ -1480,9 +1496,9 @@ int winds_march_set_surrogate(winds_march *m, winds_mlp *net, const winds_march_
     return WINDS_OK;
 }
 
-int winds_march_surr_stats(const winds_march *m, double out[8]) {
+int winds_march_surr_stats(const winds_march *m, double out[10]) {
     if (!m || !out) return set_error(WINDS_E_ARG, "null argument");
-    for (int i = 0; i < 8; ++i) out[i] = m->stats[i];
+    for (int i = 0; i < 10; ++i) out[i] = m->stats[i];
     return WINDS_OK;
 }
 

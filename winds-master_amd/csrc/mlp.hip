@@ -35,8 +35,15 @@ __device__ __forceinline__ bf16 to_bf16(float x) {
 }
 __device__ __forceinline__ float from_bf16(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
 
-// C[M][N] = A[M][K] * B[N][K]^T; M, N multiples of 32, K a multiple of 8,
-// lda/ldb multiples of 4.  Workgroup: 4 waves, a 64x64 block of C.
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+
+// C[M][N] = A[M][K] * B[N][K]^T; M, N multiples of 32, K a multiple of 32,
+// lda/ldb multiples of 32.  Workgroup: 4 waves, a 64x64 block of C, one
+// 32x32 block per wave.  Each 32-wide K block is fed as four MFMAs from two
+// 16-byte loads per lane and operand: lane l holds k in [16*(l/32), +16) of
+// the block, and MFMA q takes its elements 4q..4q+3 (A and B use the same
+// assignment of k to MFMA steps, which the sum over k does not see).  The
+// next block's loads are issued before the current block's MFMAs.
 __global__ __launch_bounds__(256) void gemm_nt(const bf16 *__restrict__ A, int lda,
                                                const bf16 *__restrict__ B, int ldb,
                                                float *__restrict__ C, int ldc, int M, int N,
@@ -45,13 +52,35 @@ __global__ __launch_bounds__(256) void gemm_nt(const bf16 *__restrict__ A, int l
     const int tr = blockIdx.y * 64 + (w >> 1) * 32;
     const int tc = blockIdx.x * 64 + (w & 1) * 32;
     if (tr >= M || tc >= N) return;
-    const bf16 *ap = A + (int64_t)(tr + (l & 31)) * lda + 4 * (l >> 5);
-    const bf16 *bp = B + (int64_t)(tc + (l & 31)) * ldb + 4 * (l >> 5);
+    const bf16 *ap = A + (int64_t)(tr + (l & 31)) * lda + 16 * (l >> 5);
+    const bf16 *bp = B + (int64_t)(tc + (l & 31)) * ldb + 16 * (l >> 5);
     f32x16 acc = {0};
-    for (int k = 0; k < K; k += 8) {
-        const bf16x4 a = *(const bf16x4 *)(ap + k);
-        const bf16x4 b = *(const bf16x4 *)(bp + k);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    bf16x8 a0 = *(const bf16x8 *)ap, a1 = *(const bf16x8 *)(ap + 8);
+    bf16x8 b0 = *(const bf16x8 *)bp, b1 = *(const bf16x8 *)(bp + 8);
+    for (int k = 0; k < K; k += 32) {
+        bf16x8 na0 = a0, na1 = a1, nb0 = b0, nb1 = b1;
+        if (k + 32 < K) {
+            na0 = *(const bf16x8 *)(ap + k + 32);
+            na1 = *(const bf16x8 *)(ap + k + 40);
+            nb0 = *(const bf16x8 *)(bp + k + 32);
+            nb1 = *(const bf16x8 *)(bp + k + 40);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(__builtin_shufflevector(a0, a0, 0, 1, 2, 3),
+                                                       __builtin_shufflevector(b0, b0, 0, 1, 2, 3),
+                                                       acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(__builtin_shufflevector(a0, a0, 4, 5, 6, 7),
+                                                       __builtin_shufflevector(b0, b0, 4, 5, 6, 7),
+                                                       acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(__builtin_shufflevector(a1, a1, 0, 1, 2, 3),
+                                                       __builtin_shufflevector(b1, b1, 0, 1, 2, 3),
+                                                       acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(__builtin_shufflevector(a1, a1, 4, 5, 6, 7),
+                                                       __builtin_shufflevector(b1, b1, 4, 5, 6, 7),
+                                                       acc, 0, 0, 0);
+        a0 = na0;
+        a1 = na1;
+        b0 = nb0;
+        b1 = nb1;
     }
     const int col = tc + (l & 31);
     for (int v = 0; v < 16; ++v) {

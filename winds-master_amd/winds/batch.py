@@ -57,7 +57,7 @@ class Batch:
             if isinstance(w, tuple):
                 t, s = w
             else:
-                t, s = layout.wake_targets(w), layout.wake_segments(w)
+                t, s = layout.wake_targets(w), layout.wake_segments(w, chained=True)
             tgs.append(np.asarray(t, dtype=np.float64))
             sgs.append(np.asarray(s, dtype=np.float64))
         self.nt = np.array([t.shape[1] for t in tgs], dtype=np.int64)

@@ -40,10 +40,21 @@ def uind_to_cm(u3: np.ndarray, pshape) -> np.ndarray:
     return out
 
 
-def wake_segments(wake) -> np.ndarray:
-    """Shed then trailing segments of a RotorWake, WInDS.m:150-154."""
-    return np.concatenate([segments(*wake.shed_segments()),
-                           segments(*wake.trail_segments())])
+def wake_segments(wake, chained: bool = False) -> np.ndarray:
+    """Shed then trailing segments of a RotorWake, WInDS.m:150-154.
+
+    chained=False keeps MATLAB's column-major order of each call.  chained=True
+    lists the trailing filaments line by line (blade, station, age from the
+    oldest down), the order the device march packs them in: consecutive
+    records are neighbouring filaments, as the shed filaments of one row
+    already are.  The set of segments is the same."""
+    shed = segments(*wake.shed_segments())
+    trail = segments(*wake.trail_segments())
+    if chained:
+        nst, _, na, nb = np.shape(wake.gamma_trail)
+        t3 = trail.reshape(nb, na, nst, 8)                 # [b][a][m]
+        trail = np.ascontiguousarray(t3[:, ::-1, :, :].transpose(0, 2, 1, 3).reshape(-1, 8))
+    return np.concatenate([shed, trail])
 
 
 def wake_targets(wake) -> np.ndarray:

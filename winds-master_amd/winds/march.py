@@ -33,7 +33,8 @@ class MarchSurr(C.Structure):
 
 
 SURR_STATS = ("checks", "failed_checks", "network_steps", "direct_far_steps", "last_err_nodes",
-              "last_err_ll", "max_err", "adam_steps")
+              "last_err_ll", "max_err", "adam_steps", "last_err_nodes_of_far",
+              "last_err_ll_of_far")
 
 
 def _bind(L):
@@ -186,6 +187,6 @@ class March:
                                                     C.byref(s)))
 
     def surr_stats(self) -> dict:
-        a = np.zeros(8)
+        a = np.zeros(10)
         _lib.check(self.L.winds_march_surr_stats(self._m, _dp(a)))
         return dict(zip(SURR_STATS, a.tolist()))

@@ -148,7 +148,7 @@ def wake_sample(rng, split, max_ages, nb=3, R=63.0, hub=1.5, U=11.4, dpsi_deg=5.
 
 
 def build_training_set(n_samples: int, opts, seed: int = 7, split: int = 144,
-                       target_ages: int = 108, max_ages: int = 1440, nb: int = 3,
+                       target_ages: int = 72, max_ages: int = 1440, nb: int = 3,
                        R: float = 63.0, U: float = 11.4, hub: float = 1.5,
                        dpsi_deg: float = 5.0, nst_range=(18, 31), log=None) -> TrainingSet:
     """Direct-sum samples of the far wake's velocity at near-wake nodes and
@@ -319,6 +319,23 @@ def train(net, Fn, Yn, steps: int, batch: int, lr: float = 1e-3, lr_end: float =
     return time.perf_counter() - t0, float(acc.item()) / min(100, steps)
 
 
+def train_two_phase(net, synthetic: TrainingSet, march: TrainingSet, steps: int, batch: int,
+                    log=None):
+    """Normalisation from both sets; half the steps on the synthetic wakes
+    (rate 1e-3 -> 3e-5), half on the march's own samples (3e-4 -> 1e-5).
+    Returns (in_norm, out_norm, seconds of each phase)."""
+    both = concat(synthetic, march)
+    in_norm, out_norm = fit_norm(both.F, both.Y)
+    del both
+    Fs, Ys = normalise(synthetic.F, synthetic.Y, in_norm, out_norm)
+    s1, _ = train(net, Fs, Ys, steps=steps // 2, batch=batch, lr=1e-3, lr_end=3e-5, log=log)
+    del Fs, Ys
+    Fm, Ym = normalise(march.F, march.Y, in_norm, out_norm)
+    s2, _ = train(net, Fm, Ym, steps=steps - steps // 2, batch=batch, lr=3e-4, lr_end=1e-5,
+                  log=log)
+    return in_norm, out_norm, (s1, s2)
+
+
 def predict(net, Fn, out_norm, chunk: int | None = None):
     """Network output in raw label units (u / U in the turning frame)."""
     import torch
@@ -343,7 +360,7 @@ class MarchSurrogate:
     """Configuration of winds_march_set_surrogate."""
     mode: int = 0                 # 0 network, 1 direct far sum, 2 far wake dropped
     split: int = 144              # 2 revolutions at 5 deg steps
-    target_ages: int = 108        # near targets: 1.5 revolutions
+    target_ages: int = 72         # near targets: 1 revolution
     check_every: int = 36
     train_iters: int = 40
     tol: float = 0.01
