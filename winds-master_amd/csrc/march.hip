@@ -509,8 +509,9 @@ __global__ void k_influence(Geom g, int A, const double *__restrict__ X,
     K[i * 3 + 2] = uz;
 }
 
-constexpr int kKjThreads = 256;
-constexpr int kMaxLL = 1024;     // lifting-line points handled by one workgroup
+constexpr int kKjThreads = 1024;
+constexpr int kKjWaves = kKjThreads / 64;
+constexpr int kMaxLL = 512;      // lifting-line points handled by one workgroup
 constexpr int kMaxNodesBlade = 256;
 
 // Kutta-Joukowski fixed point (KuttaJoukowski.m:42-49 with kj() :204-296)
@@ -535,6 +536,7 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
     __shared__ double s_g[kMaxLL];      // bound circulation
     __shared__ double s_gam[3 * kMaxLL + 4 * kMaxNodesBlade];  // changing circulation by s
     __shared__ double s_c1[kMaxLL];     // span cumsum of trailing age 1
+    __shared__ double s_u[3 * kMaxLL];  // induced velocity at the lifting line
     __shared__ double s_red[kKjThreads];
     __shared__ double s_red2[kKjThreads];
     __shared__ int s_nan;
@@ -575,19 +577,38 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
             }
         }
         __syncthreads();
-        double gnew_i[kMaxLL / kKjThreads];
+        // the changing filaments' velocity at lifting-line point i: one wave
+        // per point, lanes striding over the filaments, then a wave sum
+        {
+            const int wave = tid >> 6, lane = tid & 63;
+            for (int i = wave; i < NLL; i += kKjWaves) {
+                const double *Ki = K + (int64_t)i * NCH * 3;
+                double ux = 0.0, uy = 0.0, uz = 0.0;
+                for (int s = lane; s < NCH; s += 64) {
+                    const double gm = s_gam[s];
+                    ux += gm * Ki[3 * s];
+                    uy += gm * Ki[3 * s + 1];
+                    uz += gm * Ki[3 * s + 2];
+                }
+                for (int off = 32; off > 0; off >>= 1) {
+                    ux += __shfl_down(ux, off);
+                    uy += __shfl_down(uy, off);
+                    uz += __shfl_down(uz, off);
+                }
+                if (lane == 0) {
+                    s_u[i] = frozen[i] + ux;
+                    s_u[NLL + i] = frozen[NLL + i] + uy;
+                    s_u[2 * NLL + i] = frozen[2 * NLL + i] + uz;
+                }
+            }
+        }
+        __syncthreads();
+        double gnew_i[(kMaxLL + kKjThreads - 1) / kKjThreads];
         double local_max = 0.0;
         int q = 0;
         for (int i = tid; i < NLL; i += kKjThreads, ++q) {
             const int m = i % ns, b = i / ns;
-            double ux = frozen[i], uy = frozen[NLL + i], uz = frozen[2 * NLL + i];
-            const double *Ki = K + (int64_t)i * NCH * 3;
-            for (int s = 0; s < NCH; ++s) {
-                const double gm = s_gam[s];
-                ux += gm * Ki[3 * s];
-                uy += gm * Ki[3 * s + 1];
-                uz += gm * Ki[3 * s + 2];
-            }
+            const double ux = s_u[i], uy = s_u[NLL + i], uz = s_u[2 * NLL + i];
             const LL L = ll_frame(g, m, b, t);
             const double rx = L.bx[0] * ux + L.bx[1] * uy + L.bx[2] * uz;
             const double ry = L.by[0] * ux + L.by[1] * uy + L.by[2] * uz;

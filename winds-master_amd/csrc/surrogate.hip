@@ -1,9 +1,9 @@
 // surrogate.hip -- inputs, targets and outputs of the far-wake network.
 //
 // The far wake (every segment with an endpoint of age >= split) is reduced
-// to a small descriptor by one workgroup; each query point then becomes one
-// FP32 input row: its position in the frame turning with blade 0, followed
-// by the descriptor.  Network outputs are induced velocities in that frame
+// to a small descriptor, one workgroup per age bin; each query point then
+// becomes one FP32 input row: its position in the frame turning with blade 0,
+// followed by the descriptor.  Network outputs are induced velocities in that frame
 // divided by the free-stream speed.
 
 #include <hip/hip_runtime.h>
@@ -21,49 +21,48 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// One workgroup per age bin; workgroup 0 also writes the global entries.
 __global__ __launch_bounds__(kThreads) void k_desc(Geo g, const double *__restrict__ X,
                                                    const double *__restrict__ gt,
                                                    float *__restrict__ desc) {
     const int nst = g.nst, nb = g.nb, A = g.ages, S = g.split;
     const int64_t N = (int64_t)nst * A * nb;
     const int w = max(1, (g.max_ages - S + kBins - 1) / kBins);
+    const int k = blockIdx.x;
     __shared__ double red[kPerBin][kThreads];
-    for (int k = 0; k < kBins; ++k) {
-        const int lo = S + k * w;
-        const int hi = min(S + (k + 1) * w, A - 1);
-        const int na = max(0, hi - lo);
-        const int count = na * nb;
-        double acc[kPerBin] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int idx = threadIdx.x; idx < count; idx += kThreads) {
-            const int b = idx / na, a = lo + idx % na;
-            const int64_t j = (nst - 1) + (int64_t)a * nst + (int64_t)b * nst * A;
-            const int64_t t0 = (int64_t)a * nst + (int64_t)b * nst * (A - 1);
-            acc[0] += gt[t0 + nst - 1];
-            acc[1] += gt[t0];
-            const double y = X[N + j], z = X[2 * N + j];
-            acc[2] += X[j];
-            acc[3] += sqrt(y * y + z * z);
-            acc[4] += y;
-        }
-        for (int q = 0; q < kPerBin; ++q) red[q][threadIdx.x] = acc[q];
-        __syncthreads();
-        for (int s = kThreads / 2; s > 0; s >>= 1) {
-            if (threadIdx.x < s)
-                for (int q = 0; q < kPerBin; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-            const double inv = count > 0 ? 1.0 / count : 0.0;
-            const double gs = 10.0 / (g.U * g.R);
-            desc[k * kPerBin + 0] = (float)(red[0][0] * inv * gs);
-            desc[k * kPerBin + 1] = (float)(red[1][0] * inv * gs);
-            desc[k * kPerBin + 2] = (float)(red[2][0] * inv / g.R);
-            desc[k * kPerBin + 3] = (float)(red[3][0] * inv / g.R);
-            desc[k * kPerBin + 4] = (float)(red[4][0] * inv / g.R);
-        }
+    const int lo = S + k * w;
+    const int hi = min(S + (k + 1) * w, A - 1);
+    const int na = max(0, hi - lo);
+    const int count = na * nb;
+    double acc[kPerBin] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int idx = threadIdx.x; idx < count; idx += kThreads) {
+        const int b = idx / na, a = lo + idx % na;
+        const int64_t j = (nst - 1) + (int64_t)a * nst + (int64_t)b * nst * A;
+        const int64_t t0 = (int64_t)a * nst + (int64_t)b * nst * (A - 1);
+        acc[0] += gt[t0 + nst - 1];
+        acc[1] += gt[t0];
+        const double y = X[N + j], z = X[2 * N + j];
+        acc[2] += X[j];
+        acc[3] += sqrt(y * y + z * z);
+        acc[4] += y;
+    }
+    for (int q = 0; q < kPerBin; ++q) red[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s)
+            for (int q = 0; q < kPerBin; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
         __syncthreads();
     }
     if (threadIdx.x == 0) {
+        const double inv = count > 0 ? 1.0 / count : 0.0;
+        const double gs = 10.0 / (g.U * g.R);
+        desc[k * kPerBin + 0] = (float)(red[0][0] * inv * gs);
+        desc[k * kPerBin + 1] = (float)(red[1][0] * inv * gs);
+        desc[k * kPerBin + 2] = (float)(red[2][0] * inv / g.R);
+        desc[k * kPerBin + 3] = (float)(red[3][0] * inv / g.R);
+        desc[k * kPerBin + 4] = (float)(red[4][0] * inv / g.R);
+    }
+    if (k == 0 && threadIdx.x == 0) {
         float *gl = desc + kBins * kPerBin;
         gl[0] = (float)cos(g.psi);
         gl[1] = (float)sin(g.psi);
@@ -130,7 +129,7 @@ unsigned nblk(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 hipError_t launch_features(const Geo &g, const double *X, const double *gt, const double *P,
                            int64_t np, const float *norm, float *desc, float *feat,
                            hipStream_t st) {
-    k_desc<<<1, kThreads, 0, st>>>(g, X, gt, desc);
+    k_desc<<<kBins, kThreads, 0, st>>>(g, X, gt, desc);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || np == 0) return e;
     k_features<<<nblk(np), kThreads, 0, st>>>(g, desc, P, np, norm, feat);

@@ -27,7 +27,7 @@ inline Geo from_abi(const winds_surr_geom &g) {
 
 int check_geo(const Geo &g);
 
-// k_desc (one workgroup) then one input row per point
+// the descriptor (one workgroup per age bin), then one input row per point
 hipError_t launch_features(const Geo &g, const double *X, const double *gt, const double *P,
                            int64_t np, const float *norm, float *desc, float *feat,
                            hipStream_t st);
