@@ -29,6 +29,10 @@ FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector (packed) peak
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA peak
 FP64_PEAK_TFLOPS = 78.6
 FLOP_VATISTAS2 = 59        # algorithmic FLOPs per interaction, Vatistas n=2 form
+# HBM bytes per launch of bs_fp32c<2,1,false> on this workload: FETCH_SIZE /
+# 0.500 + WRITE_SIZE / 1.000 (rocprofv3 --pmc, factors from the plain
+# streaming kernel bs_reduce_split), profiles/bs_fp32c_vatistas2_single_wake_133920x263427.md
+TRAFFIC_BYTES = 644_621 * 1024
 FLOP_CUTOFF = 47           # cut-off (MEX) form
 
 
@@ -535,7 +539,8 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES,
+            "traffic_unit": "bytes per launch",
             "kernel": "bs_fp32c<2,1>",
             "flop_per_interaction": FLOP_VATISTAS2,
         },
