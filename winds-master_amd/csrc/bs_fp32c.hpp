@@ -19,7 +19,6 @@ __device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, int core_kind, i
         // Padding record: far away, zero strength, never near.  Every FP32
         // intermediate stays finite, so its term is exactly zero.
         r.ax = r.ay = r.az = 1.0e6;
-        r.bx = r.by = r.bz = 1.0e6;
         r.lx = r.ly = r.lz = 0.0f;
         r.near2b = -1.0f;
         r.il2 = 0.0f;
@@ -34,9 +33,6 @@ __device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, int core_kind, i
     r.ax = s.ax;
     r.ay = s.ay;
     r.az = s.az;
-    r.bx = s.bx;
-    r.by = s.by;
-    r.bz = s.bz;
     r.lx = (float)lx;
     r.ly = (float)ly;
     r.lz = (float)lz;
@@ -98,14 +94,14 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
         for (int j = 0; j < kSegTile; ++j) {
             const int64_t k = kt + j;
             const Seg32 s = s32[k];
+            // r1 = P - A in FP64, rounded once (coordinates reach ~1 km while
+            // the pairs evaluated here are metres apart).  r2 = r1 - L in
+            // FP32: every pair that reaches this arithmetic has
+            // |r2| >= |L|/2 (the near test below sends closer ones to FP64),
+            // so |r1| <= 3|r2| and r2 carries at most ~6 ulp of relative error.
             const float r1x = (float)(px - s.ax), r1y = (float)(py - s.ay),
                         r1z = (float)(pz - s.az);
-#if WINDS_R2_FROM_L
             const float r2x = r1x - s.lx, r2y = r1y - s.ly, r2z = r1z - s.lz;
-#else
-            const float r2x = (float)(px - s.bx), r2y = (float)(py - s.by),
-                        r2z = (float)(pz - s.bz);
-#endif
             const float s1 = fmaf(r1z, r1z, fmaf(r1y, r1y, r1x * r1x));
             const float s2 = fmaf(r2z, r2z, fmaf(r2y, r2y, r2x * r2x));
             const float dot = fmaf(r1z, r2z, fmaf(r1y, r2y, r1x * r2x));

@@ -13,8 +13,8 @@
 //                    268-282); bitwise equal to the CPU restatement.
 //   bs_fp64_matlab   FP64, BiotSavart.m arithmetic (:86-115) and its nested
 //                    sum order sum(sum(sum(.,1),3),4).
-//   bs_fp32c         FP32 pair arithmetic with r1 = P-A and r2 = P-B taken in
-//                    FP64 and rounded once, r1 x r2 evaluated as L x r1 (no
+//   bs_fp32c         FP32 pair arithmetic with r1 = P-A taken in FP64 and
+//                    rounded once, r2 = r1 - L, r1 x r2 evaluated as L x r1 (no
 //                    cancellation), per-tile partial sums merged into a
 //                    (hi, lo) two-sum accumulator; pairs whose target lies
 //                    within the near radius of the segment are evaluated in

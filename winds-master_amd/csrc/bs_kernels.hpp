@@ -15,17 +15,12 @@ namespace winds {
 #endif
 constexpr int kSegTile = WINDS_SEG_TILE;
 
-// r2 = P - B formed in FP64 and rounded (0), or as r1 - L in FP32 (1).
-#ifndef WINDS_R2_FROM_L
-#define WINDS_R2_FROM_L 0
-#endif
-
-// Prepared segment record of the FP32 kernel (80 bytes, read with scalar
-// loads).  The endpoints stay FP64 so that r1 = P-A and r2 = P-B are formed
-// in FP64 and rounded once; everything else is segment-only FP32.
-struct __attribute__((aligned(16))) Seg32 {
+// Prepared segment record of the FP32 kernel (64 bytes, one scalar load).
+// The start point stays FP64 so that r1 = P-A is formed in FP64 and rounded
+// once (the FP64 near-pair branch reads the Seg64 record); everything else
+// is segment-only FP32.
+struct __attribute__((aligned(64))) Seg32 {
     double ax, ay, az;   // F1 endpoint
-    double bx, by, bz;   // F2 endpoint
     float lx, ly, lz;    // L = B - A, rounded once
     float near2b;        // near2 + |L|^2 (second half of the near test)
     float il2;           // 1/|L|^2 (0 for a zero-length segment)
@@ -33,7 +28,7 @@ struct __attribute__((aligned(16))) Seg32 {
     float core;          // cut-off: d |L|^2; Vatistas: rc^n (rc = r_c^2)
     float near2;         // squared near radius; +inf: always FP64, <0: never
 };
-static_assert(sizeof(Seg32) == 80, "Seg32 layout");
+static_assert(sizeof(Seg32) == 64, "Seg32 layout");
 static_assert(sizeof(Seg64) == 64, "Seg64 layout");
 
 struct EvalCfg {
