@@ -32,7 +32,7 @@ FLOP_VATISTAS2 = 59        # algorithmic FLOPs per interaction, Vatistas n=2 for
 # HBM bytes per launch of bs_fp32c<2,1,false> on this workload: FETCH_SIZE /
 # 0.500 + WRITE_SIZE / 1.000 (rocprofv3 --pmc, factors from the plain
 # streaming kernel bs_reduce_split), profiles/bs_fp32c_vatistas2_single_wake_133920x263427.md
-TRAFFIC_BYTES = 644_621 * 1024
+TRAFFIC_BYTES = 571_957 * 1024
 FLOP_CUTOFF = 47           # cut-off (MEX) form
 
 
@@ -247,7 +247,8 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
     return rep, net, in_norm, out_norm
 
 
-def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train_steps: int):
+def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train_steps: int,
+                check_every: int = 36):
     """Free-wake time march of a 3-blade, 31-node rotor at 5 deg steps with
     PCC (2 wake evaluations per step), an FP64 Kutta-Joukowski solve and a
     1,440-age wake:
@@ -265,7 +266,7 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
     from winds.march import March
 
     rot = rmod.synthetic_rotor(nb=3, nst=31)
-    sc = S.MarchSurrogate()
+    sc = S.MarchSurrogate(check_every=check_every)
     per_rev = int(round(360 / rot.dpsi_deg))
     out = {"rotor": "3 blades x 31 nodes, R 63 m, TSR 7, U 11.4 m/s, 5 deg steps, PCC",
            "max_ages": 1440, "develop_revs": dev_revs}
@@ -395,6 +396,8 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the far-wake network")
     ap.add_argument("--surrogate-samples", type=int, default=2_000_000)
     ap.add_argument("--surrogate-steps", type=int, default=8000)
+    ap.add_argument("--check-every", type=int, default=36,
+                    help="surrogate steps between direct-sum checks")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 2 GiB device-to-device copy (reference point for "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
@@ -502,7 +505,8 @@ def main():
     march, surr_rep = None, None
     if not args.no_march:
         march, surr_rep = march_bench(args.march_revs, args.develop_revs, not args.no_surrogate,
-                                      args.surrogate_samples, args.surrogate_steps)
+                                      args.surrogate_samples, args.surrogate_steps,
+                                      args.check_every)
 
     achieved = inter * FLOP_VATISTAS2 / (ms * 1e-3) / 1e12
     line = {
