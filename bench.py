@@ -248,7 +248,7 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
 
 
 def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train_steps: int,
-                check_every: int = 36):
+                check_every: int = 72):
     """Free-wake time march of a 3-blade, 31-node rotor at 5 deg steps with
     PCC (2 wake evaluations per step), an FP64 Kutta-Joukowski solve and a
     1,440-age wake:
@@ -396,7 +396,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true", help="skip the far-wake network")
     ap.add_argument("--surrogate-samples", type=int, default=2_000_000)
     ap.add_argument("--surrogate-steps", type=int, default=8000)
-    ap.add_argument("--check-every", type=int, default=36,
+    ap.add_argument("--check-every", type=int, default=72,
                     help="surrogate steps between direct-sum checks")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one plain 2 GiB device-to-device copy (reference point for "

@@ -361,7 +361,7 @@ class MarchSurrogate:
     mode: int = 0                 # 0 network, 1 direct far sum, 2 far wake dropped
     split: int = 144              # 2 revolutions at 5 deg steps
     target_ages: int = 72         # near targets: 1 revolution
-    check_every: int = 36
+    check_every: int = 72         # one direct-sum check per revolution
     train_iters: int = 40
     tol: float = 0.01
     lr: float = 1e-4
