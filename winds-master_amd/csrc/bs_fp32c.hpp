@@ -114,7 +114,7 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
             // Conservative near test: every target closer than R to the
             // closed segment has h2 < R^2 and min(s1, s2) < R^2 + |L|^2.
             // NaN anywhere selects the FP64 branch.
-            const bool near = !(h2 >= s.near2) && !(fminf(s1, s2) >= s.near2b);
+            const bool near = !(h2 >= s.near2) && !(s1 >= s.near2b && s2 >= s.near2b);
             const float i1 = __builtin_amdgcn_rsqf(s1);
             const float i2 = __builtin_amdgcn_rsqf(s2);
             const float pp = i1 * i2;

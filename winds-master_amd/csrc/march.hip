@@ -584,11 +584,24 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
             for (int i = wave; i < NLL; i += kKjWaves) {
                 const double *Ki = K + (int64_t)i * NCH * 3;
                 double ux = 0.0, uy = 0.0, uz = 0.0;
-                for (int s = lane; s < NCH; s += 64) {
-                    const double gm = s_gam[s];
-                    ux += gm * Ki[3 * s];
-                    uy += gm * Ki[3 * s + 1];
-                    uz += gm * Ki[3 * s + 2];
+                // four filaments per lane per pass, their loads issued together
+                for (int s0 = lane; s0 < NCH; s0 += 4 * 64) {
+                    double kv[4][3], gv[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int s = s0 + 64 * q;
+                        const int sc = s < NCH ? s : 0;
+                        gv[q] = s < NCH ? s_gam[sc] : 0.0;
+                        kv[q][0] = Ki[3 * sc];
+                        kv[q][1] = Ki[3 * sc + 1];
+                        kv[q][2] = Ki[3 * sc + 2];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        ux += gv[q] * kv[q][0];
+                        uy += gv[q] * kv[q][1];
+                        uz += gv[q] * kv[q][2];
+                    }
                 }
                 for (int off = 32; off > 0; off >>= 1) {
                     ux += __shfl_down(ux, off);
