@@ -452,7 +452,7 @@ int winds_batch_autotune(winds_batch *b, const double *targets, const double *se
     if (!b) return set_error(WINDS_E_ARG, "null plan");
     hipStream_t st = (hipStream_t)stream;
     const int blocks[] = {256, 128, 64};
-    const int chunks[] = {4096, 8192, 16384, 32768};
+    const int chunks[] = {2048, 4096, 8192, 16384, 32768};
     const winds_batch_tune start = b->tune;
     winds_batch_tune win = start;
     double win_ms = 1e300;
