@@ -62,7 +62,8 @@ typedef struct winds_bs_opts {
     double co;             /* far cut-off distance (SEM_MATLAB); +inf = none */
     double near_factor;    /* FP32C: FP64 for pairs with the target within   */
     double near_len_frac;  /*   max(near_factor*r_c, near_len_frac*|L|) of   */
-                           /*   the segment; <= 0 selects 40 and 1/2         */
+                           /*   the segment; <= 0 selects 40 and 0.51;       */
+                           /*   fractions below 0.51 act as 0.51             */
     int32_t block;         /* threads per block; 0 = tuned                   */
     int32_t split;         /* segment-range split; FP32C: 0 = tuned; FP64:  */
                            /*   0/1 = one sequential sum (reference order),  */

@@ -49,7 +49,7 @@ def test_version_and_defaults():
     o = _lib.BsOpts()
     L.winds_bs_opts_default(C.byref(o))
     assert o.core == 0 and o.semantics == 0 and o.precision == 0
-    assert o.d == 1e-4 and math.isinf(o.co) and o.near_factor == 40.0 and o.near_len_frac == 0.5
+    assert o.d == 1e-4 and math.isinf(o.co) and o.near_factor == 40.0 and o.near_len_frac == 0.51
 
 
 def _call(type="fuln", cmod="none", d=1e-4, fd=(2, 3, 2, 1), pd=(2, 3, 2, 1), gpuhw=(256, 1, 0)):

@@ -24,13 +24,16 @@ namespace {
 thread_local std::string g_err;
 
 // Default near radius of the FP32 kernel: pairs whose target lies within
-// max(40 r_c, |L|/2) of the segment are evaluated in FP64.  Chosen by
+// max(40 r_c, 0.51 |L|) of the segment are evaluated in FP64.  Chosen by
 // measurement on the 133,920-target single-wake workload and on smaller
-// wakes of other TSR, yaw and size: per-target
-// relative error below 1e-6 against FP64 for the cut-off and Vatistas cores,
-// for about 5% of the evaluation time.
+// wakes of other TSR, yaw and size: per-target relative error below 1e-6
+// against FP64 for the cut-off and Vatistas cores, for about 9% of the
+// evaluation time (20 r_c: 6% faster, max error 7.0e-7 on the single wake;
+// 10 r_c: 1.2e-6).  0.51 |L| is also the least fraction the FP32 arithmetic
+// allows (every target inside the sphere on the segment as diameter must be
+// near).
 constexpr double kNearFactor = 40.0;
-constexpr double kNearLenFrac = 0.5;
+constexpr double kNearLenFrac = 0.51;
 
 int fail(int code, const std::string &msg) {
     g_err = msg;

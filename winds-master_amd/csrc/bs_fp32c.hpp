@@ -11,6 +11,21 @@
 
 namespace winds {
 
+// Smallest near radius as a fraction of |L|: with R >= 0.51|L| every target
+// inside the sphere on the segment as diameter (r1.r2 <= 0) is near, so the
+// FP32 arithmetic never meets r1 r2 + r1.r2 ~ 0 (0.5 plus rounding margin).
+constexpr double kMinNearLenFrac = 0.51;
+
+// Copy of a wave-uniform value into a VGPR.  gfx950 issues FP32 VALU
+// operations whose operands are all VGPRs at twice the rate of those that
+// read an SGPR, and a move takes one fast slot: a record value feeding
+// several products is moved once.
+__device__ __forceinline__ float to_vgpr(float s) {
+    float v;
+    asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
+    return v;
+}
+
 // Prepared record k from FP64 record s, or a padding record when s is null.
 __device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, int core_kind, int n, double d,
                                             double near_factor, double near_len_frac) {
@@ -41,7 +56,8 @@ __device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, int core_kind, i
     const double core = core_kind == 0 ? d * l2 : ipow(s.rc, n);
     r.core = (float)core;
     double nr2 = near_factor * near_factor * s.rc;
-    const double nl2 = near_len_frac * near_len_frac * l2;
+    const double nlf = near_len_frac > kMinNearLenFrac ? near_len_frac : kMinNearLenFrac;
+    const double nl2 = nlf * nlf * l2;
     if (nl2 > nr2) nr2 = nl2;
     if (!(nr2 > 0.0) || !(s.g == s.g)) nr2 = __builtin_inf();
     r.near2 = (float)nr2;
@@ -95,6 +111,8 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
         for (int j = 0; j < kSegTile; ++j) {
             const int64_t k = kt + j;
             const Seg32 s = s32[k];
+            // L feeds three products per component: one move to a VGPR each
+            const float Lx = to_vgpr(s.lx), Ly = to_vgpr(s.ly), Lz = to_vgpr(s.lz);
             // r1 = P - A in FP64, rounded once (coordinates reach ~1 km while
             // the pairs evaluated here are metres apart).  r2 = r1 - L in
             // FP32: every pair that reaches this arithmetic has
@@ -102,60 +120,50 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
             // so |r1| <= 3|r2| and r2 carries at most ~6 ulp of relative error.
             const float r1x = (float)(px - s.ax), r1y = (float)(py - s.ay),
                         r1z = (float)(pz - s.az);
-            const float r2x = r1x - s.lx, r2y = r1y - s.ly, r2z = r1z - s.lz;
+            const float r2x = r1x - Lx, r2y = r1y - Ly, r2z = r1z - Lz;
             const float s1 = fmaf(r1z, r1z, fmaf(r1y, r1y, r1x * r1x));
             const float s2 = fmaf(r2z, r2z, fmaf(r2y, r2y, r2x * r2x));
             const float dot = fmaf(r1z, r2z, fmaf(r1y, r2y, r1x * r2x));
             // r1 x r2 = L x r1 with L = B - A.
-            const float cx = fmaf(s.ly, r1z, -(s.lz * r1y));
-            const float cy = fmaf(s.lz, r1x, -(s.lx * r1z));
-            const float cz = fmaf(s.lx, r1y, -(s.ly * r1x));
+            const float cx = fmaf(Ly, r1z, -(Lz * r1y));
+            const float cy = fmaf(Lz, r1x, -(Lx * r1z));
+            const float cz = fmaf(Lx, r1y, -(Ly * r1x));
             const float c2 = fmaf(cz, cz, fmaf(cy, cy, cx * cx));  // |r1 x r2|^2
             const float h2 = c2 * s.il2;  // squared distance to the segment's line
             // Conservative near test: every target closer than R to the
             // closed segment has h2 < R^2 and min(s1, s2) < R^2 + |L|^2.
-            // NaN anywhere selects the FP64 branch.
-            const bool near = !(h2 >= s.near2) && !(s1 >= s.near2b && s2 >= s.near2b);
+            // NaN anywhere selects the FP64 branch.  R >= 0.51|L| makes every
+            // target inside the sphere on the segment as diameter (the pairs
+            // with r1.r2 <= 0) near, so below r1.r2 > 0 and r1 r2 + r1.r2
+            // never cancels.
+            const bool near = !(h2 >= s.near2) && !(fminf(s1, s2) >= s.near2b);
             const float i1 = __builtin_amdgcn_rsqf(s1);
             const float i2 = __builtin_amdgcn_rsqf(s2);
             const float pp = i1 * i2;
-            const float r1r2 = (s1 * s2) * pp;
-            // r1 r2 + r1.r2 cancels when the target sits beside the segment
-            // (r1.r2 < 0); there it equals |r1 x r2|^2 / (r1 r2 - r1.r2),
-            // whose terms are both accurate.  qa = r1 r2 + |r1.r2| never cancels.
-            const float qa = r1r2 + fabsf(dot);
-            const bool side = dot <= 0.0f;
-            const float num = i1 + i2;
+            const float qa = fmaf(s1 * s2, pp, dot);  // r1 r2 + r1.r2
+            const float gn = s.g * (i1 + i2);
             float ubar;
             if constexpr (CORE == 0) {
-                // 1/(r1r2 + dot + d|L|^2/(r1r2)), in the form matching side
-                const float t1 = s.core * pp;
-                const float nume = side ? qa : 1.0f;
-                const float deno = side ? fmaf(t1, qa, c2) : (qa + t1);
-                ubar = ((s.g * num) * nume) * __builtin_amdgcn_rcpf(deno);
+                // 1/(r1r2 + r1.r2 + d|L|^2/(r1r2))
+                ubar = gn * __builtin_amdgcn_rcpf(fmaf(s.core, pp, qa));
+            } else if constexpr (CORE == 1) {
+                // h^2/(r_c^2 + h^2) / (r1r2 + r1.r2)
+                const float G = fmaxf((s.core + h2) * qa, FLT_MIN);
+                ubar = gn * (h2 * __builtin_amdgcn_rcpf(G));
+            } else if constexpr (CORE == 2) {
+                // h^2/sqrt(r_c^4 + h^4) / (r1r2 + r1.r2)
+                const float w = fmaxf(fmaf(h2, h2, s.core), FLT_MIN);
+                ubar = gn * (h2 * __builtin_amdgcn_rsqf(w * (qa * qa)));
             } else {
-                // C/den with C the Vatistas factor: side: il2 qa C/h2,
-                // otherwise C/qa
-                const float F = side ? s.il2 * qa : h2;
-                if constexpr (CORE == 1) {
-                    const float w = s.core + h2;
-                    const float G = side ? w : w * qa;
-                    ubar = (s.g * num) * (F * __builtin_amdgcn_rcpf(fmaxf(G, FLT_MIN)));
-                } else if constexpr (CORE == 2) {
-                    const float w = fmaxf(fmaf(h2, h2, s.core), FLT_MIN);
-                    const float G = side ? w : w * (qa * qa);
-                    ubar = (s.g * num) * (F * __builtin_amdgcn_rsqf(G));
-                } else {
-                    float hn = h2;
-                    for (int i = 1; i < vat_n; ++i) hn = hn * h2;
-                    const float w = fmaxf(s.core + hn, FLT_MIN);
-                    const float root = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(w) * neg_inv_n);
-                    const float G = side ? 1.0f : qa;
-                    ubar = (s.g * num) * (F * root * __builtin_amdgcn_rcpf(G));
-                }
+                // h^2/(r_c^2n + h^2n)^(1/n) / (r1r2 + r1.r2)
+                float hn = h2;
+                for (int i = 1; i < vat_n; ++i) hn = hn * h2;
+                const float w = fmaxf(s.core + hn, FLT_MIN);
+                const float root = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(w) * neg_inv_n);
+                ubar = gn * (h2 * root * __builtin_amdgcn_rcpf(qa));
             }
             bool zero = near;
-            if constexpr (CO) zero = zero || (s1 > co2 && s2 > co2);
+            if constexpr (CO) zero = zero || !(fminf(s1, s2) <= co2);
             ubar = zero ? 0.0f : ubar;
             qx = fmaf(ubar, cx, qx);
             qy = fmaf(ubar, cy, qy);

@@ -39,7 +39,7 @@ struct EvalCfg {
     double d = 1e-4;
     double co = __builtin_inf();
     double near_factor = 40.0;
-    double near_len_frac = 0.5;
+    double near_len_frac = 0.51;
     int block = 0;
     int split = 0;
     int64_t seg_col = 0;

@@ -98,7 +98,7 @@ def check(code: int) -> None:
 
 def opts(core: int = CORE_CUTOFF, vat_n: int = 2, semantics: int = SEM_MEX,
          precision: int = PREC_FP64, d: float = 1e-4, co: float = math.inf,
-         near_factor: float = 40.0, near_len_frac: float = 0.5, block: int = 0,
+         near_factor: float = 40.0, near_len_frac: float = 0.51, block: int = 0,
          split: int = 0, seg_col: int = 0, seg_blk: int = 0) -> BsOpts:
     o = BsOpts()
     lib().winds_bs_opts_default(C.byref(o))
