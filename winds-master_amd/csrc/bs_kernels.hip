@@ -45,18 +45,16 @@ __global__ __launch_bounds__(1024) void bs_fp64_mex(
     double *__restrict__ uz) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
-    const double px = tx[t], py = ty[t], pz = tz[t];
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int64_t k = 0; k < nseg; ++k) {
-        const Seg64 s = seg[k];
-        mex_pair(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, px, py, pz, d, sx, sy, sz);
-    }
+    fp64_range<0, 0>(seg, 0, nseg, tx[t], ty[t], tz[t], d, 0, 0.0, sx, sy, sz);
     ux[t] = sx;
     uy[t] = sy;
     uz[t] = sz;
 }
 
 // col = segments per station column, blk = segments per blade (col * ages).
+// MODE 1 cut-off, 2 Vatistas of order NF (1, 2) or n (NF = 0).
+template <int MODE, int NF>
 __global__ __launch_bounds__(1024) void bs_fp64_matlab(
     const double *__restrict__ tx, const double *__restrict__ ty,
     const double *__restrict__ tz, int64_t nt, const Seg64 *__restrict__ seg,
@@ -72,19 +70,7 @@ __global__ __launch_bounds__(1024) void bs_fp64_matlab(
         for (int64_t c0 = b0; c0 < b1; c0 += col) {
             const int64_t c1 = c0 + col < b1 ? c0 + col : b1;
             double cx = 0.0, cy = 0.0, cz = 0.0;
-            for (int64_t k = c0; k < c1; ++k) {
-                const Seg64 s = seg[k];
-                double ex, ey, ez;
-                if (n != 0)
-                    matlab_pair<true>(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, s.rc, px, py, pz,
-                                      d, n, co, ex, ey, ez);
-                else
-                    matlab_pair<false>(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, s.rc, px, py, pz,
-                                       d, 0, co, ex, ey, ez);
-                cx += ex;
-                cy += ey;
-                cz += ez;
-            }
+            fp64_range<MODE, NF>(seg, c0, c1, px, py, pz, d, n, co, cx, cy, cz);
             bx += cx;
             by += cy;
             bz += cz;
@@ -100,9 +86,10 @@ __global__ __launch_bounds__(1024) void bs_fp64_matlab(
 
 // FP64 over one segment chunk per grid.y (split-K for few targets); the
 // partial of chunk y goes to part[y][3][nt].  MODE: 0 biot_p2p (MEX),
-// 1 BiotSavart.m cut-off, 2 BiotSavart.m Vatistas order n.  Segments are
-// summed sequentially within a chunk; chunks are added by bs_reduce_split.
-template <int MODE>
+// 1 BiotSavart.m cut-off, 2 BiotSavart.m Vatistas order NF (1, 2) or n
+// (NF = 0).  Segments are summed sequentially within a chunk; chunks are
+// added by bs_reduce_split.
+template <int MODE, int NF>
 __global__ __launch_bounds__(256) void bs_fp64_split(
     const double *__restrict__ tx, const double *__restrict__ ty,
     const double *__restrict__ tz, int64_t nt, const Seg64 *__restrict__ seg,
@@ -111,21 +98,8 @@ __global__ __launch_bounds__(256) void bs_fp64_split(
     if (t >= nt) return;
     const int64_t k0 = (int64_t)blockIdx.y * chunk;
     const int64_t k1 = k0 + chunk < nseg ? k0 + chunk : nseg;
-    const double px = tx[t], py = ty[t], pz = tz[t];
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int64_t k = k0; k < k1; ++k) {
-        const Seg64 s = seg[k];
-        if constexpr (MODE == 0) {
-            mex_pair(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, px, py, pz, d, sx, sy, sz);
-        } else {
-            double ex, ey, ez;
-            matlab_pair<MODE == 2>(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, s.rc, px, py, pz, d,
-                                   n, co, ex, ey, ez);
-            sx += ex;
-            sy += ey;
-            sz += ez;
-        }
-    }
+    fp64_range<MODE, NF>(seg, k0, k1, tx[t], ty[t], tz[t], d, n, co, sx, sy, sz);
     double *o = part + (int64_t)blockIdx.y * 3 * nt;
     o[t] = sx;
     o[nt + t] = sy;
@@ -276,12 +250,20 @@ hipError_t launch_fp64(const double *t3, int64_t nt, const Seg64 *seg,
         const int nsplit = (int)((nseg + chunk - 1) / chunk);
         dim3 grid(grid_for(nt, block), nsplit);
         const int n = c.core == 0 ? 0 : c.vat_n;
+#define WINDS_SPLIT(M, F)                                                                 \
+    bs_fp64_split<M, F><<<grid, block, 0, st>>>(tx, ty, tz, nt, seg, nseg, chunk, c.d, n, c.co, \
+                                                part)
         if (c.semantics == 0)
-            bs_fp64_split<0><<<grid, block, 0, st>>>(tx, ty, tz, nt, seg, nseg, chunk, c.d, 0, c.co, part);
+            WINDS_SPLIT(0, 0);
         else if (n == 0)
-            bs_fp64_split<1><<<grid, block, 0, st>>>(tx, ty, tz, nt, seg, nseg, chunk, c.d, 0, c.co, part);
+            WINDS_SPLIT(1, 0);
+        else if (n == 1)
+            WINDS_SPLIT(2, 1);
+        else if (n == 2)
+            WINDS_SPLIT(2, 2);
         else
-            bs_fp64_split<2><<<grid, block, 0, st>>>(tx, ty, tz, nt, seg, nseg, chunk, c.d, n, c.co, part);
+            WINDS_SPLIT(2, 0);
+#undef WINDS_SPLIT
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         bs_reduce_split<<<grid_for(nt, 256), 256, 0, st>>>(part, nt, nsplit, ux, uy, uz);
@@ -293,8 +275,18 @@ hipError_t launch_fp64(const double *t3, int64_t nt, const Seg64 *seg,
         const int64_t col = c.seg_col > 0 ? c.seg_col : nseg;
         const int64_t blk = c.seg_blk > 0 ? c.seg_blk : nseg;
         const int n = c.core == 0 ? 0 : c.vat_n;
-        bs_fp64_matlab<<<grid_for(nt, block), block, 0, st>>>(tx, ty, tz, nt, seg, nseg, col, blk,
-                                                              c.d, n, c.co, ux, uy, uz);
+#define WINDS_MATLAB(M, F)                                                                   \
+    bs_fp64_matlab<M, F><<<grid_for(nt, block), block, 0, st>>>(tx, ty, tz, nt, seg, nseg, col, \
+                                                                blk, c.d, n, c.co, ux, uy, uz)
+        if (n == 0)
+            WINDS_MATLAB(1, 0);
+        else if (n == 1)
+            WINDS_MATLAB(2, 1);
+        else if (n == 2)
+            WINDS_MATLAB(2, 2);
+        else
+            WINDS_MATLAB(2, 0);
+#undef WINDS_MATLAB
     }
     return hipGetLastError();
 }

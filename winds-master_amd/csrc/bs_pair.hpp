@@ -22,11 +22,12 @@ constexpr double kMlPi = 3.141592653589793;
 // with d*|L|^2 in the denominator; rc and co unused; nothing zeroed.
 // x1,y1,z1 = F1 endpoint, x2,y2,z2 = F2 endpoint (the reference packs them
 // as fx = (x1, x2), BiotSavartMex.cu:347-352).
-__device__ __forceinline__ void mex_pair(double x1, double y1, double z1,
+// The term alone; mex_pair adds it.
+__device__ __forceinline__ void mex_term(double x1, double y1, double z1,
                                          double x2, double y2, double z2,
                                          double g, double px, double py,
-                                         double pz, double cd, double &ux,
-                                         double &uy, double &uz) {
+                                         double pz, double cd, double &tx,
+                                         double &ty, double &tz) {
     double ldx = x1 - x2;
     double ldy = y1 - y2;
     double ldz = z1 - z2;
@@ -44,9 +45,21 @@ __device__ __forceinline__ void mex_pair(double x1, double y1, double z1,
     double den = r1tr2 * (r1tr2 + r1dr2) + cd * len;
     double ubar = (g * (r1 + r2)) * (1.000 / (4.000 * kMexPi));
     ubar = ubar / den;
-    ux += ubar * (pyy1 * pzz2 - pzz1 * pyy2);
-    uy += ubar * (pzz1 * pxx2 - pxx1 * pzz2);
-    uz += ubar * (pxx1 * pyy2 - pyy1 * pxx2);
+    tx = ubar * (pyy1 * pzz2 - pzz1 * pyy2);
+    ty = ubar * (pzz1 * pxx2 - pxx1 * pzz2);
+    tz = ubar * (pxx1 * pyy2 - pyy1 * pxx2);
+}
+
+__device__ __forceinline__ void mex_pair(double x1, double y1, double z1,
+                                         double x2, double y2, double z2,
+                                         double g, double px, double py,
+                                         double pz, double cd, double &ux,
+                                         double &uy, double &uz) {
+    double tx, ty, tz;
+    mex_term(x1, y1, z1, x2, y2, z2, g, px, py, pz, cd, tx, ty, tz);
+    ux += tx;
+    uy += ty;
+    uz += tz;
 }
 
 __device__ __forceinline__ double ipow(double x, int n) {
@@ -113,6 +126,51 @@ __device__ __forceinline__ void matlab_pair(double x1, double y1, double z1,
     tx = ubar * (pyy1 * pzz2 - pzz1 * pyy2);
     ty = ubar * (pzz1 * pxx2 - pxx1 * pzz2);
     tz = ubar * (pxx1 * pyy2 - pyy1 * pxx2);
+}
+
+// Term of segment record s at (px, py, pz).  MODE 0: biot_p2p (MEX);
+// 1: BiotSavart.m cut-off; 2: BiotSavart.m Vatistas, order NF (1, 2) or n
+// (NF = 0).
+template <int MODE, int NF>
+__device__ __forceinline__ void fp64_term(const Seg64 &s, double px, double py, double pz,
+                                          double d, int n, double co, double &tx, double &ty,
+                                          double &tz) {
+    if constexpr (MODE == 0)
+        mex_term(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, px, py, pz, d, tx, ty, tz);
+    else
+        matlab_pair<MODE == 2, NF>(s.ax, s.ay, s.az, s.bx, s.by, s.bz, s.g, s.rc, px, py, pz, d,
+                                   n, co, tx, ty, tz);
+}
+
+// Adds the terms of segments [k0, k1) to (sx, sy, sz) one after another in
+// segment order -- the reference's summation, so the result is bit-identical
+// to a plain loop -- while the terms of four segments are computed side by
+// side: the FP64 pipeline has independent work while one term's square
+// roots and divisions are in flight.
+template <int MODE, int NF>
+__device__ __forceinline__ void fp64_range(const Seg64 *__restrict__ seg, int64_t k0, int64_t k1,
+                                           double px, double py, double pz, double d, int n,
+                                           double co, double &sx, double &sy, double &sz) {
+    int64_t k = k0;
+    for (; k + 4 <= k1; k += 4) {
+        double t[4][3];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            fp64_term<MODE, NF>(seg[k + i], px, py, pz, d, n, co, t[i][0], t[i][1], t[i][2]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            sx += t[i][0];
+            sy += t[i][1];
+            sz += t[i][2];
+        }
+    }
+    for (; k < k1; ++k) {
+        double tx, ty, tz;
+        fp64_term<MODE, NF>(seg[k], px, py, pz, d, n, co, tx, ty, tz);
+        sx += tx;
+        sy += ty;
+        sz += tz;
+    }
 }
 
 }  // namespace winds
