@@ -32,7 +32,7 @@ FLOP_VATISTAS2 = 59        # algorithmic FLOPs per interaction, Vatistas n=2 for
 # HBM bytes per launch of bs_fp32c<2,1,false> on this workload: FETCH_SIZE /
 # 0.500 + WRITE_SIZE / 1.000 (rocprofv3 --pmc, factors from the plain
 # streaming kernel bs_reduce_split), profiles/bs_fp32c_vatistas2_single_wake_133920x263427.md
-TRAFFIC_BYTES = 850_073 * 1024
+TRAFFIC_BYTES = 850_115 * 1024
 FLOP_CUTOFF = 47           # cut-off (MEX) form
 
 
@@ -525,7 +525,7 @@ def main():
         "config": {
             "workload": (f"single large wake, FP32 compensated vs FP64: 3 blades x 31 stations x "
                          f"{args.ages} ages, {nt} targets x {nseg} segments (shed+trail) per "
-                         f"evaluation, Vatistas n=2 core, FP64 for pairs within max(40 r_c, |L|/2)"),
+                         f"evaluation, Vatistas n=2 core, FP64 for pairs within max(40 r_c, 0.51 |L|)"),
             "targets": nt, "segments": nseg, "interactions_per_step": inter,
             "core": "vatistas2", "precision": "fp32 compensated + fp64 near pairs",
         },
