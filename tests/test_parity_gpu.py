@@ -107,10 +107,19 @@ def test_fp32c_config1_whole_wake(c1, core, chained):
         o = winds.opts(core=winds.CORE_VATISTAS, vat_n=n, semantics=winds.SEM_MATLAB,
                        precision=winds.PREC_FP32C)
         ref = O.bs_matlab(F1, F2, c1.domain, recs[:, 6], recs[:, 7], f"visc{n}")
-    u = layout.uind_to_cm(Evaluator(0).eval(tg, sg, o).cpu().numpy(), c1.domain.shape)
+    ev = Evaluator(0)
+    u = layout.uind_to_cm(ev.eval(tg, sg, o).cpu().numpy(), c1.domain.shape)
     s = stats(u, ref)
     flush_print(f"[fp32c whole wake {core} chained={chained}] {s}")
     assert s["max_rel"] <= 1e-6 and s["rms_rel"] <= 1e-7 and s["normwise"] <= 1e-7
+    # segment ranges that start inside a filament (the kernel then forms r1
+    # from the range's first start point instead of carrying it)
+    for split in (2, 5, 7):
+        o.split = split
+        us = layout.uind_to_cm(ev.eval(tg, sg, o).cpu().numpy(), c1.domain.shape)
+        ss = stats(us, ref)
+        flush_print(f"   split {split}: {ss}")
+        assert ss["max_rel"] <= 1e-6 and ss["rms_rel"] <= 1e-7 and ss["normwise"] <= 1e-7
 
 
 # --------------------------------------------------------------------------
@@ -227,6 +236,43 @@ def test_device_api_matches_host_api(c1):
         assert s["normwise"] <= 1e-6
     for x in outs[1:]:
         assert np.max(np.abs(x - outs[0])) <= 1e-7 * np.max(np.abs(outs[0]))
+
+
+@pytest.mark.parametrize("core", ["vatistas2", "cutoff"])
+def test_fp32c_targets_inside_segment_sphere_take_fp64(need_gpu, core):
+    """A target inside the sphere on a segment as diameter (r1.r2 < 0) is a
+    near pair of the FP32 kernel: the near radius is at least 0.51|L|, also
+    when a smaller fraction is asked for.  Its term is then the FP64 term,
+    bit for bit, and the FP32 arithmetic never meets r1 r2 + r1.r2 ~ 0."""
+    import torch
+    from winds.device import Evaluator
+    ev = Evaluator(0)
+    if core == "cutoff":
+        kw = dict(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, d=1e-4)
+    else:
+        kw = dict(core=winds.CORE_VATISTAS, vat_n=2, semantics=winds.SEM_MATLAB)
+    rng = np.random.default_rng(11)
+    n = 4096
+    for trial in range(6):
+        A = rng.uniform(-800.0, 800.0, 3)
+        L = rng.normal(size=3) * rng.uniform(0.2, 20.0)
+        F1 = A.reshape(1, 3, 1, 1)
+        F2 = (A + L).reshape(1, 3, 1, 1)
+        g = np.array([rng.uniform(-60.0, 60.0)])
+        rc = np.array([rng.uniform(1e-4, 1e-2)])  # squared core radius
+        sg = torch.from_numpy(layout.segments(F1, F2, g, rc)).cuda()
+        d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rho = 0.999 * rng.uniform(0.0, 1.0, n) ** (1.0 / 3.0)
+        P = (A + 0.5 * L) + 0.5 * np.linalg.norm(L) * rho[:, None] * d
+        assert np.all(np.einsum("ij,ij->i", P - A, P - A - L) < 0.0)
+        tg = torch.from_numpy(np.ascontiguousarray(P.T)).cuda()
+        flush_print(f"[inside sphere {core}] trial {trial}, |L| = {np.linalg.norm(L):.3f}")
+        u64 = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP64, **kw)).cpu().numpy()
+        for frac in (0.51, 0.3):
+            o = winds.opts(precision=winds.PREC_FP32C, near_factor=1e-3, near_len_frac=frac, **kw)
+            u32 = ev.eval(tg, sg, o).cpu().numpy()
+            assert np.array_equal(u32, u64), (trial, frac, np.max(np.abs(u32 - u64)))
 
 
 # --------------------------------------------------------------------------

@@ -26,35 +26,42 @@ __device__ __forceinline__ float to_vgpr(float s) {
     return v;
 }
 
-// Prepared record k from FP64 record s, or a padding record when s is null.
-__device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, int core_kind, int n, double d,
-                                            double near_factor, double near_len_frac) {
+// Prepared record from FP64 record s (prev: the record before it in the same
+// set, or null), or a padding record when s is null.
+__device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, const Seg64 *prev, int core_kind,
+                                            int n, double d, double near_factor,
+                                            double near_len_frac) {
     Seg32 r;
     if (!s_ptr) {
-        // Padding record: far away, zero strength, never near.  Every FP32
-        // intermediate stays finite, so its term is exactly zero.
-        r.ax = r.ay = r.az = 1.0e6;
+        r.bx = r.by = r.bz = 1.0e6;
         r.lx = r.ly = r.lz = 0.0f;
         r.near2b = -1.0f;
         r.il2 = 0.0f;
         r.g = 0.0f;
         r.core = 1.0f;
         r.near2 = -1.0f;
+        r.flags = kSegSkip;
+        r.unused = 0;
         return r;
     }
     const Seg64 s = *s_ptr;
     const double lx = s.bx - s.ax, ly = s.by - s.ay, lz = s.bz - s.az;
     const double l2 = lx * lx + ly * ly + lz * lz;
-    r.ax = s.ax;
-    r.ay = s.ay;
-    r.az = s.az;
+    r.bx = s.bx;
+    r.by = s.by;
+    r.bz = s.bz;
     r.lx = (float)lx;
     r.ly = (float)ly;
     r.lz = (float)lz;
     r.il2 = l2 > 0.0 ? (float)(1.0 / l2) : 0.0f;
     r.g = l2 > 0.0 ? (float)(s.g / (4.0 * kMlPi)) : 0.0f;
-    const double core = core_kind == 0 ? d * l2 : ipow(s.rc, n);
-    r.core = (float)core;
+    if (core_kind == 0) {
+        r.core = (float)(d * l2);
+    } else {
+        // FLT_MIN keeps r_c^4 + h^4 > 0 without a per-pair max
+        const float c = (float)ipow(s.rc, n);
+        r.core = c > FLT_MIN ? c : FLT_MIN;
+    }
     double nr2 = near_factor * near_factor * s.rc;
     const double nlf = near_len_frac > kMinNearLenFrac ? near_len_frac : kMinNearLenFrac;
     const double nl2 = nlf * nlf * l2;
@@ -62,6 +69,8 @@ __device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, int core_kind, i
     if (!(nr2 > 0.0) || !(s.g == s.g)) nr2 = __builtin_inf();
     r.near2 = (float)nr2;
     r.near2b = (float)(nr2 + l2);
+    r.flags = (prev && prev->bx == s.ax && prev->by == s.ay && prev->bz == s.az) ? kSegChained : 0;
+    r.unused = 0;
     return r;
 }
 
@@ -82,10 +91,113 @@ struct Fp32Params {
     double co;        // far cut-off (near pairs)
 };
 
+// FP32 view of P - X for a filament point X: the vector, its squared length
+// and the reciprocal of its length.
+struct RelPoint {
+    float x, y, z, s, i;
+};
+
+__device__ __forceinline__ RelPoint rel_point(double px, double py, double pz, double x, double y,
+                                              double z) {
+    // formed in FP64 and rounded once: coordinates reach ~1 km while the
+    // pairs evaluated in FP32 are metres apart
+    RelPoint r;
+    r.x = (float)(px - x);
+    r.y = (float)(py - y);
+    r.z = (float)(pz - z);
+    r.s = fmaf(r.z, r.z, fmaf(r.y, r.y, r.x * r.x));
+    r.i = __builtin_amdgcn_rsqf(r.s);
+    return r;
+}
+
+// One record: the term of segment k (start point a, end point b) added to
+// (qx, qy, qz) in FP32, or to (ax, ay, az) by the FP64 reference arithmetic
+// for a near pair.  Returns P - B for the next record of the filament.
+template <int CORE, int SEM, bool CO>
+__device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz, const Seg32 &s,
+                                                 const Seg64 *__restrict__ q64, RelPoint a,
+                                                 const Fp32Params &P, float &qx, float &qy,
+                                                 float &qz, double &ax, double &ay, double &az) {
+    if (!(s.flags & kSegChained)) a = rel_point(px, py, pz, q64->ax, q64->ay, q64->az);
+    const RelPoint b = rel_point(px, py, pz, s.bx, s.by, s.bz);
+    // L feeds three products per component: one move to a VGPR each
+    const float Lx = to_vgpr(s.lx), Ly = to_vgpr(s.ly), Lz = to_vgpr(s.lz);
+    const float dot = fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x));
+    // r1 x r2 = L x r1 with L = B - A.
+    const float cx = fmaf(Ly, a.z, -(Lz * a.y));
+    const float cy = fmaf(Lz, a.x, -(Lx * a.z));
+    const float cz = fmaf(Lx, a.y, -(Ly * a.x));
+    const float c2 = fmaf(cz, cz, fmaf(cy, cy, cx * cx));  // |r1 x r2|^2
+    const float h2 = c2 * s.il2;  // squared distance to the segment's line
+    // Conservative near test: every target closer than R to the closed
+    // segment has h2 < R^2 and min(r1^2, r2^2) < R^2 + |L|^2.  NaN anywhere
+    // selects the FP64 branch.  R >= 0.51|L| makes every target inside the
+    // sphere on the segment as diameter (the pairs with r1.r2 <= 0) near, so
+    // below r1.r2 > 0 and r1 r2 + r1.r2 never cancels.
+    const bool near = !(h2 >= s.near2) & (!(a.s >= s.near2b) | !(b.s >= s.near2b));
+    const float pp = a.i * b.i;
+    const float qa = fmaf(a.s * b.s, pp, dot);  // r1 r2 + r1.r2
+    const float gn = s.g * (a.i + b.i);
+    float ubar;
+    if constexpr (CORE == 0) {
+        // 1/(r1r2 + r1.r2 + d|L|^2/(r1r2))
+        ubar = gn * __builtin_amdgcn_rcpf(fmaf(s.core, pp, qa));
+    } else if constexpr (CORE == 1) {
+        // h^2/(r_c^2 + h^2) / (r1r2 + r1.r2)
+        const float G = fmaxf((s.core + h2) * qa, FLT_MIN);
+        ubar = gn * (h2 * __builtin_amdgcn_rcpf(G));
+    } else if constexpr (CORE == 2) {
+        // h^2/sqrt(r_c^4 + h^4) / (r1r2 + r1.r2); r_c^4 >= FLT_MIN
+        const float w = fmaf(h2, h2, s.core);
+        ubar = gn * (h2 * __builtin_amdgcn_rsqf(w * (qa * qa)));
+    } else {
+        // h^2/(r_c^2n + h^2n)^(1/n) / (r1r2 + r1.r2)
+        float hn = h2;
+        for (int i = 1; i < P.vat_n; ++i) hn = hn * h2;
+        const float w = fmaxf(s.core + hn, FLT_MIN);
+        const float root = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(w) * P.neg_inv_n);
+        ubar = gn * (h2 * root * __builtin_amdgcn_rcpf(qa));
+    }
+    bool zero = near;
+    if constexpr (CO) zero = zero | (!(a.s <= P.co2) & !(b.s <= P.co2));
+    ubar = zero ? 0.0f : ubar;
+    qx = fmaf(ubar, cx, qx);
+    qy = fmaf(ubar, cy, qy);
+    qz = fmaf(ubar, cz, qz);
+    if (near) {
+        const Seg64 q = *q64;
+        if constexpr (SEM == 0) {
+            mex_pair(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, px, py, pz, P.d, ax, ay, az);
+        } else {
+            double ex, ey, ez;
+            matlab_pair<CORE != 0, (CORE == 1 || CORE == 2) ? CORE : 0>(
+                q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, q.rc, px, py, pz, P.d, P.vat_n, P.co, ex,
+                ey, ez);
+            ax += ex;
+            ay += ey;
+            az += ez;
+        }
+    }
+    return b;
+}
+
+// Record k with all sixteen words in SGPRs before the first branch on its
+// flags: one batch of scalar loads and one wait per record (left alone, the
+// compiler loads the flags first and the rest under the branch, paying the
+// scalar-cache latency twice).
+__device__ __forceinline__ Seg32 load_record(const Seg32 *__restrict__ s32, int64_t k) {
+    Seg32 s = s32[k];
+    asm volatile("" : "+s"(s.flags)
+                 : "s"(s.bx), "s"(s.by), "s"(s.bz), "s"(s.lx), "s"(s.ly), "s"(s.lz),
+                   "s"(s.near2b), "s"(s.il2), "s"(s.g), "s"(s.core), "s"(s.near2));
+    return s;
+}
+
 // Sum of the terms of segments [k0, k1) (prepared records, k1 - k0 a
 // multiple of kSegTile) at target (px, py, pz).  s64 + shift is the FP64
 // record of prepared record k.  Every lane of the wave must walk the same
-// range: the records are wave-uniform scalar loads.
+// range: the records are wave-uniform scalar loads, and so is every branch
+// on their flags.
 //
 // CORE: 0 cut-off, 1 Vatistas n=1, 2 Vatistas n=2, 3 Vatistas general n.
 // SEM:  0 MEX semantics (near pairs by biot_p2p), 1 BiotSavart.m semantics
@@ -97,90 +209,31 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                                             const Seg64 *__restrict__ s64, int64_t shift,
                                             int64_t k0, int64_t k1, const Fp32Params &P,
                                             double &ox, double &oy, double &oz) {
-    const float co2 = P.co2;
-    const float neg_inv_n = P.neg_inv_n;
-    const int vat_n = P.vat_n;
-    const double d = P.d;
-    const double co = P.co;
     float hx = 0.f, hy = 0.f, hz = 0.f, lx = 0.f, ly = 0.f, lz = 0.f;
     double ax = 0.0, ay = 0.0, az = 0.0;
+    // P - (end point of the record before), carried along the filament; the
+    // range's first record forms it from its own start point
+    RelPoint e{0.f, 0.f, 0.f, 1.f, 1.f};
+    if (k0 < k1 && !(s32[k0].flags & kSegSkip))
+        e = rel_point(px, py, pz, s64[k0 + shift].ax, s64[k0 + shift].ay, s64[k0 + shift].az);
     for (int64_t kt = k0; kt < k1; kt += kSegTile) {
         float qx = 0.f, qy = 0.f, qz = 0.f;
-        // not unrolled: measured 9% faster than two records per iteration
+        // two records per iteration: the carried point alternates between
+        // two sets of registers instead of being copied
 #pragma unroll 1
-        for (int j = 0; j < kSegTile; ++j) {
+        for (int j = 0; j < kSegTile; j += 2) {
             const int64_t k = kt + j;
-            const Seg32 s = s32[k];
-            // L feeds three products per component: one move to a VGPR each
-            const float Lx = to_vgpr(s.lx), Ly = to_vgpr(s.ly), Lz = to_vgpr(s.lz);
-            // r1 = P - A in FP64, rounded once (coordinates reach ~1 km while
-            // the pairs evaluated here are metres apart).  r2 = r1 - L in
-            // FP32: every pair that reaches this arithmetic has
-            // |r2| >= |L|/2 (the near test below sends closer ones to FP64),
-            // so |r1| <= 3|r2| and r2 carries at most ~6 ulp of relative error.
-            const float r1x = (float)(px - s.ax), r1y = (float)(py - s.ay),
-                        r1z = (float)(pz - s.az);
-            const float r2x = r1x - Lx, r2y = r1y - Ly, r2z = r1z - Lz;
-            const float s1 = fmaf(r1z, r1z, fmaf(r1y, r1y, r1x * r1x));
-            const float s2 = fmaf(r2z, r2z, fmaf(r2y, r2y, r2x * r2x));
-            const float dot = fmaf(r1z, r2z, fmaf(r1y, r2y, r1x * r2x));
-            // r1 x r2 = L x r1 with L = B - A.
-            const float cx = fmaf(Ly, r1z, -(Lz * r1y));
-            const float cy = fmaf(Lz, r1x, -(Lx * r1z));
-            const float cz = fmaf(Lx, r1y, -(Ly * r1x));
-            const float c2 = fmaf(cz, cz, fmaf(cy, cy, cx * cx));  // |r1 x r2|^2
-            const float h2 = c2 * s.il2;  // squared distance to the segment's line
-            // Conservative near test: every target closer than R to the
-            // closed segment has h2 < R^2 and min(s1, s2) < R^2 + |L|^2.
-            // NaN anywhere selects the FP64 branch.  R >= 0.51|L| makes every
-            // target inside the sphere on the segment as diameter (the pairs
-            // with r1.r2 <= 0) near, so below r1.r2 > 0 and r1 r2 + r1.r2
-            // never cancels.
-            const bool near = !(h2 >= s.near2) && !(fminf(s1, s2) >= s.near2b);
-            const float i1 = __builtin_amdgcn_rsqf(s1);
-            const float i2 = __builtin_amdgcn_rsqf(s2);
-            const float pp = i1 * i2;
-            const float qa = fmaf(s1 * s2, pp, dot);  // r1 r2 + r1.r2
-            const float gn = s.g * (i1 + i2);
-            float ubar;
-            if constexpr (CORE == 0) {
-                // 1/(r1r2 + r1.r2 + d|L|^2/(r1r2))
-                ubar = gn * __builtin_amdgcn_rcpf(fmaf(s.core, pp, qa));
-            } else if constexpr (CORE == 1) {
-                // h^2/(r_c^2 + h^2) / (r1r2 + r1.r2)
-                const float G = fmaxf((s.core + h2) * qa, FLT_MIN);
-                ubar = gn * (h2 * __builtin_amdgcn_rcpf(G));
-            } else if constexpr (CORE == 2) {
-                // h^2/sqrt(r_c^4 + h^4) / (r1r2 + r1.r2)
-                const float w = fmaxf(fmaf(h2, h2, s.core), FLT_MIN);
-                ubar = gn * (h2 * __builtin_amdgcn_rsqf(w * (qa * qa)));
-            } else {
-                // h^2/(r_c^2n + h^2n)^(1/n) / (r1r2 + r1.r2)
-                float hn = h2;
-                for (int i = 1; i < vat_n; ++i) hn = hn * h2;
-                const float w = fmaxf(s.core + hn, FLT_MIN);
-                const float root = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(w) * neg_inv_n);
-                ubar = gn * (h2 * root * __builtin_amdgcn_rcpf(qa));
-            }
-            bool zero = near;
-            if constexpr (CO) zero = zero || !(fminf(s1, s2) <= co2);
-            ubar = zero ? 0.0f : ubar;
-            qx = fmaf(ubar, cx, qx);
-            qy = fmaf(ubar, cy, qy);
-            qz = fmaf(ubar, cz, qz);
-            if (near) {
-                const Seg64 q = s64[k + shift];
-                if constexpr (SEM == 0) {
-                    mex_pair(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, px, py, pz, d, ax, ay, az);
-                } else {
-                    double ex, ey, ez;
-                    matlab_pair<CORE != 0, (CORE == 1 || CORE == 2) ? CORE : 0>(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, q.rc, px, py,
-                                           pz, d, vat_n, co, ex, ey, ez);
-                    ax += ex;
-                    ay += ey;
-                    az += ez;
-                }
-            }
+            const Seg32 sa = load_record(s32, k);
+            RelPoint f = e;
+            if (!(sa.flags & kSegSkip))
+                f = fp32c_record<CORE, SEM, CO>(px, py, pz, sa, s64 + k + shift, e, P, qx, qy, qz,
+                                                ax, ay, az);
+            const Seg32 sb = load_record(s32, k + 1);
+            if (!(sb.flags & kSegSkip))
+                e = fp32c_record<CORE, SEM, CO>(px, py, pz, sb, s64 + k + 1 + shift, f, P, qx, qy,
+                                                qz, ax, ay, az);
+            else
+                e = f;
         }
         two_sum_acc(hx, lx, qx);
         two_sum_acc(hy, ly, qy);

@@ -15,19 +15,26 @@ namespace winds {
 #endif
 constexpr int kSegTile = WINDS_SEG_TILE;
 
-// Prepared segment record of the FP32 kernel (64 bytes, one scalar load).
-// The start point stays FP64 so that r1 = P-A is formed in FP64 and rounded
-// once (the FP64 near-pair branch reads the Seg64 record); everything else
-// is segment-only FP32.
+// Prepared segment record of the FP32 kernel (64 bytes, scalar loads).
+// The end point B stays FP64 so that r2 = P-B is formed in FP64 and rounded
+// once.  Filaments are chains (a trailing line, a shed line): when segment
+// k starts where segment k-1 ends, r1 of segment k is r2 of segment k-1 and
+// the kernel carries it (with |r|^2 and 1/|r|) from one record to the next;
+// otherwise it forms r1 from the Seg64 record's start point.  The FP64
+// near-pair branch reads the Seg64 record.
 struct __attribute__((aligned(64))) Seg32 {
-    double ax, ay, az;   // F1 endpoint
+    double bx, by, bz;   // F2 endpoint
     float lx, ly, lz;    // L = B - A, rounded once
     float near2b;        // near2 + |L|^2 (second half of the near test)
     float il2;           // 1/|L|^2 (0 for a zero-length segment)
     float g;             // gamma / (4 pi) (0 for a zero-length segment)
-    float core;          // cut-off: d |L|^2; Vatistas: rc^n (rc = r_c^2)
+    float core;          // cut-off: d |L|^2; Vatistas: max(rc^n, FLT_MIN) (rc = r_c^2)
     float near2;         // squared near radius; +inf: always FP64, <0: never
+    int32_t flags;       // kSegChained, kSegSkip
+    int32_t unused;
 };
+constexpr int32_t kSegChained = 1;  // starts at the previous record's end point
+constexpr int32_t kSegSkip = 2;     // padding: no term
 static_assert(sizeof(Seg32) == 64, "Seg32 layout");
 static_assert(sizeof(Seg64) == 64, "Seg64 layout");
 
