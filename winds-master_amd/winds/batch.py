@@ -83,8 +83,11 @@ class Batch:
     def __del__(self):
         plan = getattr(self, "_plan", None)
         if plan is not None and plan.value:
-            self.L.winds_batch_destroy(plan)
-            self._plan = C.c_void_p()
+            try:
+                self.L.winds_batch_destroy(plan)
+                plan.value = None
+            except Exception:       # interpreter shutdown
+                pass
 
     def info(self) -> dict:
         a = (C.c_int64 * 4)()
