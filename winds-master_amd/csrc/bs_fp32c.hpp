@@ -110,51 +110,65 @@ __device__ __forceinline__ RelPoint rel_point(double px, double py, double pz, d
     return r;
 }
 
-// One record: the term of segment k (start point a, end point b) added to
-// (qx, qy, qz) in FP32, or to (ax, ay, az) by the FP64 reference arithmetic
-// for a near pair.  Returns P - B for the next record of the filament.
+// The FP32 fields of a record as the pair arithmetic reads them: each either
+// left in its SGPR (one slow operand) or moved to a VGPR once (one fast move)
+// when it feeds more than one operation.
+struct RecF {
+    float lx, ly, lz, near2b, il2, g, core, near2;
+};
+
+// L (three uses per component) in VGPRs, the rest SGPR operands.
+__device__ __forceinline__ RecF rec_fields_1(const Seg32 &s) {
+    return RecF{to_vgpr(s.lx), to_vgpr(s.ly), to_vgpr(s.lz), s.near2b, s.il2, s.g, s.core,
+                s.near2};
+}
+
+// One record: the term of segment k (start point a, end point b, FP32
+// fields f) added to (qx, qy, qz) in FP32, or to (ax, ay, az) by the FP64
+// reference arithmetic for a near pair.  Returns P - B for the next record
+// of the filament.
 template <int CORE, int SEM, bool CO>
 __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz, const Seg32 &s,
-                                                 const Seg64 *__restrict__ q64, RelPoint a,
-                                                 const Fp32Params &P, float &qx, float &qy,
-                                                 float &qz, double &ax, double &ay, double &az) {
+                                                 const RecF &f, const Seg64 *__restrict__ q64,
+                                                 RelPoint a, const Fp32Params &P, float &qx,
+                                                 float &qy, float &qz, double &ax, double &ay,
+                                                 double &az) {
     if (!(s.flags & kSegChained)) a = rel_point(px, py, pz, q64->ax, q64->ay, q64->az);
     const RelPoint b = rel_point(px, py, pz, s.bx, s.by, s.bz);
-    // L feeds three products per component: one move to a VGPR each
-    const float Lx = to_vgpr(s.lx), Ly = to_vgpr(s.ly), Lz = to_vgpr(s.lz);
+    const float Lx = f.lx, Ly = f.ly, Lz = f.lz;
     const float dot = fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x));
     // r1 x r2 = L x r1 with L = B - A.
     const float cx = fmaf(Ly, a.z, -(Lz * a.y));
     const float cy = fmaf(Lz, a.x, -(Lx * a.z));
     const float cz = fmaf(Lx, a.y, -(Ly * a.x));
     const float c2 = fmaf(cz, cz, fmaf(cy, cy, cx * cx));  // |r1 x r2|^2
-    const float h2 = c2 * s.il2;  // squared distance to the segment's line
+    const float h2 = c2 * f.il2;  // squared distance to the segment's line
     // Conservative near test: every target closer than R to the closed
     // segment has h2 < R^2 and min(r1^2, r2^2) < R^2 + |L|^2.  NaN anywhere
     // selects the FP64 branch.  R >= 0.51|L| makes every target inside the
     // sphere on the segment as diameter (the pairs with r1.r2 <= 0) near, so
     // below r1.r2 > 0 and r1 r2 + r1.r2 never cancels.
-    const bool near = !(h2 >= s.near2) & (!(a.s >= s.near2b) | !(b.s >= s.near2b));
+    const bool near = !(h2 >= f.near2) & (!(a.s >= f.near2b) | !(b.s >= f.near2b));
     const float pp = a.i * b.i;
     const float qa = fmaf(a.s * b.s, pp, dot);  // r1 r2 + r1.r2
-    const float gn = s.g * (a.i + b.i);
+    const float gn = f.g * (a.i + b.i);
     float ubar;
     if constexpr (CORE == 0) {
         // 1/(r1r2 + r1.r2 + d|L|^2/(r1r2))
-        ubar = gn * __builtin_amdgcn_rcpf(fmaf(s.core, pp, qa));
+        ubar = gn * __builtin_amdgcn_rcpf(fmaf(f.core, pp, qa));
     } else if constexpr (CORE == 1) {
         // h^2/(r_c^2 + h^2) / (r1r2 + r1.r2)
-        const float G = fmaxf((s.core + h2) * qa, FLT_MIN);
+        const float G = fmaxf((f.core + h2) * qa, FLT_MIN);
         ubar = gn * (h2 * __builtin_amdgcn_rcpf(G));
     } else if constexpr (CORE == 2) {
         // h^2/sqrt(r_c^4 + h^4) / (r1r2 + r1.r2); r_c^4 >= FLT_MIN
-        const float w = fmaf(h2, h2, s.core);
+        const float w = fmaf(h2, h2, f.core);
         ubar = gn * (h2 * __builtin_amdgcn_rsqf(w * (qa * qa)));
     } else {
         // h^2/(r_c^2n + h^2n)^(1/n) / (r1r2 + r1.r2)
         float hn = h2;
         for (int i = 1; i < P.vat_n; ++i) hn = hn * h2;
-        const float w = fmaxf(s.core + hn, FLT_MIN);
+        const float w = fmaxf(f.core + hn, FLT_MIN);
         const float root = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(w) * P.neg_inv_n);
         ubar = gn * (h2 * root * __builtin_amdgcn_rcpf(qa));
     }
@@ -226,12 +240,12 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
             const Seg32 sa = load_record(s32, k);
             RelPoint f = e;
             if (!(sa.flags & kSegSkip))
-                f = fp32c_record<CORE, SEM, CO>(px, py, pz, sa, s64 + k + shift, e, P, qx, qy, qz,
-                                                ax, ay, az);
+                f = fp32c_record<CORE, SEM, CO>(px, py, pz, sa, rec_fields_1(sa), s64 + k + shift,
+                                                e, P, qx, qy, qz, ax, ay, az);
             const Seg32 sb = load_record(s32, k + 1);
             if (!(sb.flags & kSegSkip))
-                e = fp32c_record<CORE, SEM, CO>(px, py, pz, sb, s64 + k + 1 + shift, f, P, qx, qy,
-                                                qz, ax, ay, az);
+                e = fp32c_record<CORE, SEM, CO>(px, py, pz, sb, rec_fields_1(sb),
+                                                s64 + k + 1 + shift, f, P, qx, qy, qz, ax, ay, az);
             else
                 e = f;
         }
