@@ -162,9 +162,10 @@ int to_cfg(const winds_bs_opts *o, winds::EvalCfg &c) {
 
 // Split of the segment range for the FP32 kernel: enough waves to keep every
 // SIMD busy at the kernel's occupancy, chosen so the last round of waves is
-// as full as possible, and chunks of at most ~2k records (short chunks even
+// as full as possible, and chunks of at most ~3k records (short chunks even
 // out the waves that take the FP64 near-pair branch; measured on the
-// 1,440-age wake: 23 splits 41.7 ms, 128 splits 39.9 ms).
+// 1,440-age wake, Vatistas-2: 32 splits 35.0 ms, 64-96 splits 34.0-34.1 ms,
+// 129 splits 34.6 ms, 258 splits 35.9 ms).
 int pick_split(int64_t nt, int64_t nseg, int block) {
     const int64_t waves = ((nt + 63) / 64);
     const int64_t tiles = winds::seg32_padded(nseg) / winds::kSegTile;
@@ -183,7 +184,7 @@ int pick_split(int64_t nt, int64_t nseg, int block) {
             best = s;
         }
     }
-    const int64_t by_chunk = std::min<int64_t>((nseg + 2047) / 2048, std::min<int64_t>(tiles / 4, 1024));
+    const int64_t by_chunk = std::min<int64_t>((nseg + 3071) / 3072, std::min<int64_t>(tiles / 4, 1024));
     return (int)std::max<int64_t>(best, by_chunk);
 }
 
