@@ -82,6 +82,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CORE == 3 ?
     const WorkItem *__restrict__ items, int n_items, int *__restrict__ counter, Fp32Params P,
     double *__restrict__ part) {
     __shared__ int s_item;
+    extern __shared__ NearLds near_lds[];  // one per wave
     const int B = blockDim.x;
     const int lane = threadIdx.x;
     for (;;) {
@@ -96,8 +97,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CORE == 3 ?
         const bool active = lane < w.tn;
         const int64_t t = w.t0 + (active ? lane : 0);
         double sx, sy, sz;
-        fp32c_range<CORE, SEM, CO>(tx[t], ty[t], tz[t], s32, s64, w.shift, w.k0, w.k1, P, sx, sy,
-                                   sz);
+        fp32c_range<CORE, SEM, CO>(tx[t], ty[t], tz[t], s32, s64, w.shift, w.k0, w.k1, P,
+                                   near_lds + (lane >> 6), sx, sy, sz);
         if (active) {
             double *o = part + (int64_t)w.slot * 3 * B;
             o[lane] = sx;
@@ -184,7 +185,9 @@ template <int K, int S, bool C>
 int occ_blocks(int block) {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, winds::bs_batch_persistent<K, S, C>, block,
-                                                     0) != hipSuccess || n < 1)
+                                                     (size_t)(block / 64) *
+                                                         sizeof(winds::NearLds)) != hipSuccess ||
+        n < 1)
         n = std::max(1, 16 / std::max(1, block / 64));
     return n;
 }
@@ -285,7 +288,9 @@ template <int K, int S, bool C>
 void launch_persistent(const winds_batch *b, const double *t3, const winds::Seg32 *s32,
                        const winds::Seg64 *s64, const winds::Fp32Params &P, double *part,
                        hipStream_t st) {
-    winds::bs_batch_persistent<K, S, C><<<b->grid, b->tune.block, 0, st>>>(
+    winds::bs_batch_persistent<K, S, C><<<b->grid, b->tune.block,
+                                          (size_t)(b->tune.block / 64) * sizeof(winds::NearLds),
+                                          st>>>(
         t3, t3 + b->nt_total, t3 + 2 * b->nt_total, s32, s64, b->d_items, (int)b->n_items,
         b->d_counter, P, part);
 }

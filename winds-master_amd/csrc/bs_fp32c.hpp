@@ -26,6 +26,14 @@ __device__ __forceinline__ float to_vgpr(float s) {
     return v;
 }
 
+// Orders the wave's LDS accesses: LDS operations of one wave execute in
+// issue order; this keeps the compiler from moving them across each other.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Prepared record from FP64 record s (prev: the record before it in the same
 // set, or null), or a padding record when s is null.
 __device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, const Seg64 *prev, int core_kind,
@@ -123,6 +131,101 @@ __device__ __forceinline__ RecF rec_fields_1(const Seg32 &s) {
                 s.near2};
 }
 
+// Near pairs go to a per-wave list in LDS instead of a branch in the pair
+// loop: a branch is paid by the whole wave (about 170 FP64 instructions for
+// the reference arithmetic) however few lanes are near.  Every near lane
+// appends its (record, lane) pair; when the list is full and at the end of
+// the range, the wave evaluates the listed pairs 64 at a time, one per
+// lane, and adds each term to its target's FP64 sum in LDS.  The order of
+// those additions is fixed (list order, then lane order within one LDS
+// instruction), so the result is the same on every run.
+constexpr int kNearCap = 256;  // pairs per wave between evaluations
+
+struct NearLds {
+    uint32_t pair[kNearCap];  // (k - k0) * 64 + lane
+    double p[3][64];          // the wave's targets
+    double acc[3][64];        // their FP64 near-pair sums
+};
+
+template <int CORE, int SEM>
+struct NearList {
+    NearLds *L;
+    const Seg64 *s64;  // FP64 record of prepared record k: s64[k + shift]
+    int64_t shift;
+    int64_t k0;
+    int n;             // pairs listed (wave-uniform)
+
+    __device__ __forceinline__ void init(NearLds *l, const Seg64 *q, int64_t sh, int64_t kbase,
+                                         double px, double py, double pz) {
+        L = l;
+        s64 = q;
+        shift = sh;
+        k0 = kbase;
+        n = 0;
+        const int lane = (int)(threadIdx.x & 63u);
+        wave_lds_sync();
+        L->p[0][lane] = px;
+        L->p[1][lane] = py;
+        L->p[2][lane] = pz;
+        L->acc[0][lane] = 0.0;
+        L->acc[1][lane] = 0.0;
+        L->acc[2][lane] = 0.0;
+        wave_lds_sync();
+    }
+
+    // Evaluates the listed pairs (called with the whole wave).
+    __device__ __forceinline__ void flush(const Fp32Params &P) {
+        wave_lds_sync();
+        const int lane = (int)(threadIdx.x & 63u);
+        for (int base = 0; base < n; base += 64) {
+            const int i = base + lane;
+            if (i < n) {
+                const uint32_t e = L->pair[i];
+                const int tl = (int)(e & 63u);
+                const Seg64 q = s64[k0 + (int64_t)(e >> 6) + shift];
+                const double px = L->p[0][tl], py = L->p[1][tl], pz = L->p[2][tl];
+                double ex, ey, ez;
+                if constexpr (SEM == 0) {
+                    mex_term(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, px, py, pz, P.d, ex, ey,
+                             ez);
+                } else {
+                    matlab_pair<CORE != 0, (CORE == 1 || CORE == 2) ? CORE : 0>(
+                        q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, q.rc, px, py, pz, P.d, P.vat_n,
+                        P.co, ex, ey, ez);
+                }
+                atomicAdd(&L->acc[0][tl], ex);
+                atomicAdd(&L->acc[1][tl], ey);
+                atomicAdd(&L->acc[2][tl], ez);
+            }
+        }
+        wave_lds_sync();
+        n = 0;
+    }
+
+    // Lists the pair (k, lane) of every lane of mask m (m = ballot(mine)).
+    // The caller keeps room for 64 pairs per record between flushes.
+    __device__ __forceinline__ void add(uint64_t m, bool mine, int64_t k) {
+        const int c = __popcll(m);
+        if (mine) {
+            const int lane = (int)(threadIdx.x & 63u);
+            const int below = (int)__builtin_amdgcn_mbcnt_hi(
+                (uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            L->pair[n + below] = (uint32_t)((k - k0) * 64 + lane);
+        }
+        n += c;
+    }
+
+    // The sums of this lane's near pairs (after a final flush).
+    __device__ __forceinline__ void result(const Fp32Params &P, double &ax, double &ay,
+                                           double &az) {
+        flush(P);
+        const int lane = (int)(threadIdx.x & 63u);
+        ax = L->acc[0][lane];
+        ay = L->acc[1][lane];
+        az = L->acc[2][lane];
+    }
+};
+
 // One record: the term of segment k (start point a, end point b, FP32
 // fields f) added to (qx, qy, qz) in FP32, or to (ax, ay, az) by the FP64
 // reference arithmetic for a near pair.  Returns P - B for the next record
@@ -130,9 +233,9 @@ __device__ __forceinline__ RecF rec_fields_1(const Seg32 &s) {
 template <int CORE, int SEM, bool CO>
 __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz, const Seg32 &s,
                                                  const RecF &f, const Seg64 *__restrict__ q64,
-                                                 RelPoint a, const Fp32Params &P, float &qx,
-                                                 float &qy, float &qz, double &ax, double &ay,
-                                                 double &az) {
+                                                 int64_t k, RelPoint a, const Fp32Params &P,
+                                                 float &qx, float &qy, float &qz,
+                                                 NearList<CORE, SEM> &nl) {
     if (!(s.flags & kSegChained)) a = rel_point(px, py, pz, q64->ax, q64->ay, q64->az);
     const RelPoint b = rel_point(px, py, pz, s.bx, s.by, s.bz);
     const float Lx = f.lx, Ly = f.ly, Lz = f.lz;
@@ -178,20 +281,8 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     qx = fmaf(ubar, cx, qx);
     qy = fmaf(ubar, cy, qy);
     qz = fmaf(ubar, cz, qz);
-    if (near) {
-        const Seg64 q = *q64;
-        if constexpr (SEM == 0) {
-            mex_pair(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, px, py, pz, P.d, ax, ay, az);
-        } else {
-            double ex, ey, ez;
-            matlab_pair<CORE != 0, (CORE == 1 || CORE == 2) ? CORE : 0>(
-                q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, q.rc, px, py, pz, P.d, P.vat_n, P.co, ex,
-                ey, ez);
-            ax += ex;
-            ay += ey;
-            az += ez;
-        }
-    }
+    const uint64_t m = __ballot(near);
+    if (m) nl.add(m, near, k);
     return b;
 }
 
@@ -222,9 +313,10 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                                             const Seg32 *__restrict__ s32,
                                             const Seg64 *__restrict__ s64, int64_t shift,
                                             int64_t k0, int64_t k1, const Fp32Params &P,
-                                            double &ox, double &oy, double &oz) {
+                                            NearLds *nlds, double &ox, double &oy, double &oz) {
     float hx = 0.f, hy = 0.f, hz = 0.f, lx = 0.f, ly = 0.f, lz = 0.f;
-    double ax = 0.0, ay = 0.0, az = 0.0;
+    NearList<CORE, SEM> nl;
+    nl.init(nlds, s64, shift, k0, px, py, pz);
     // P - (end point of the record before), carried along the filament; the
     // range's first record forms it from its own start point
     RelPoint e{0.f, 0.f, 0.f, 1.f, 1.f};
@@ -241,18 +333,23 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
             RelPoint f = e;
             if (!(sa.flags & kSegSkip))
                 f = fp32c_record<CORE, SEM, CO>(px, py, pz, sa, rec_fields_1(sa), s64 + k + shift,
-                                                e, P, qx, qy, qz, ax, ay, az);
+                                                k, e, P, qx, qy, qz, nl);
             const Seg32 sb = load_record(s32, k + 1);
             if (!(sb.flags & kSegSkip))
                 e = fp32c_record<CORE, SEM, CO>(px, py, pz, sb, rec_fields_1(sb),
-                                                s64 + k + 1 + shift, f, P, qx, qy, qz, ax, ay, az);
+                                                s64 + k + 1 + shift, k + 1, f, P, qx, qy, qz, nl);
             else
                 e = f;
+            // flushed here, between record pairs, where little else is live
+            // (inside the record it pushed the loop to 89 VGPRs, 5 waves)
+            if (nl.n > kNearCap - 128) nl.flush(P);
         }
         two_sum_acc(hx, lx, qx);
         two_sum_acc(hy, ly, qy);
         two_sum_acc(hz, lz, qz);
     }
+    double ax, ay, az;
+    nl.result(P, ax, ay, az);
     ox = ((double)hx + (double)lx) + ax;
     oy = ((double)hy + (double)ly) + ay;
     oz = ((double)hz + (double)lz) + az;

@@ -123,17 +123,24 @@ __global__ void prep_seg32(const Seg64 *__restrict__ seg, int64_t nseg,
 // One launch over a single segment set: grid.x covers targets, grid.y
 // splits the segment range; split partials go to out[split][3][stride].
 template <int CORE, int SEM, bool CO>
-__global__ __launch_bounds__(256) void bs_fp32c(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CORE == 3 ? 4 : 7))) void bs_fp32c(
     const double *__restrict__ tx, const double *__restrict__ ty,
     const double *__restrict__ tz, int64_t nt, const Seg32 *__restrict__ s32,
     const Seg64 *__restrict__ s64, int64_t nseg_pad, int64_t chunk, Fp32Params P,
     double *__restrict__ out, int64_t out_stride) {
+    extern __shared__ NearLds near_lds[];  // one per wave
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
+    // whole waves past the end leave; the lanes past the end of a partial
+    // wave take part in the wave's near-pair evaluation and store nothing
+    if (t - (int64_t)(threadIdx.x & 63u) >= nt) return;
+    const bool active = t < nt;
+    const int64_t tc = active ? t : nt - 1;
     const int64_t k0 = (int64_t)blockIdx.y * chunk;
     const int64_t k1 = k0 + chunk < nseg_pad ? k0 + chunk : nseg_pad;
     double sx, sy, sz;
-    fp32c_range<CORE, SEM, CO>(tx[t], ty[t], tz[t], s32, s64, 0, k0, k1, P, sx, sy, sz);
+    fp32c_range<CORE, SEM, CO>(tx[tc], ty[tc], tz[tc], s32, s64, 0, k0, k1, P,
+                               near_lds + (threadIdx.x >> 6), sx, sy, sz);
+    if (!active) return;
     double *o = out + (int64_t)blockIdx.y * 3 * out_stride;
     o[t] = sx;
     o[out_stride + t] = sy;
@@ -299,8 +306,9 @@ static void fp32c_dispatch(dim3 grid, int block, hipStream_t st, const double *t
                            const double *ty, const double *tz, int64_t nt,
                            const Seg32 *s32, const Seg64 *s64, int64_t npad,
                            int64_t chunk, const Fp32Params &P, double *out, int64_t stride) {
-    bs_fp32c<CORE, SEM, CO><<<grid, block, 0, st>>>(tx, ty, tz, nt, s32, s64, npad, chunk, P, out,
-                                                    stride);
+    const size_t lds = (size_t)(block / 64) * sizeof(NearLds);
+    bs_fp32c<CORE, SEM, CO><<<grid, block, lds, st>>>(tx, ty, tz, nt, s32, s64, npad, chunk, P, out,
+                                                      stride);
 }
 
 hipError_t launch_fp32c(const double *t3, int64_t nt, const Seg64 *seg,
