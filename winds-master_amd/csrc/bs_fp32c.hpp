@@ -251,7 +251,8 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     // selects the FP64 branch.  R >= 0.51|L| makes every target inside the
     // sphere on the segment as diameter (the pairs with r1.r2 <= 0) near, so
     // below r1.r2 > 0 and r1 r2 + r1.r2 never cancels.
-    const bool near = !(h2 >= f.near2) & (!(a.s >= f.near2b) | !(b.s >= f.near2b));
+    const bool n1 = !(h2 >= f.near2), n2 = !(a.s >= f.near2b), n3 = !(b.s >= f.near2b);
+    const bool near = n1 & (n2 | n3);
     const float pp = a.i * b.i;
     const float qa = fmaf(a.s * b.s, pp, dot);  // r1 r2 + r1.r2
     const float gn = f.g * (a.i + b.i);
@@ -281,7 +282,10 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     qx = fmaf(ubar, cx, qx);
     qy = fmaf(ubar, cy, qy);
     qz = fmaf(ubar, cz, qz);
-    const uint64_t m = __ballot(near);
+    // the wave's near mask from the three compare masks (a ballot of the
+    // combined condition would go through a VGPR and a fourth compare)
+    const uint64_t m = __builtin_amdgcn_ballot_w64(n1) &
+                       (__builtin_amdgcn_ballot_w64(n2) | __builtin_amdgcn_ballot_w64(n3));
     if (m) nl.add(m, near, k);
     return b;
 }
