@@ -89,7 +89,7 @@ def test_fp32c_config1(c1, cmod, d):
 
 
 @pytest.mark.parametrize("chained", [False, True])
-@pytest.mark.parametrize("core", ["vatistas2", "vatistas1", "cutoff"])
+@pytest.mark.parametrize("core", ["vatistas2", "vatistas1", "vatistas3", "cutoff"])
 def test_fp32c_config1_whole_wake(c1, core, chained):
     import torch
     from winds.device import Evaluator
