@@ -112,6 +112,10 @@ def test_fp32c_config1_whole_wake(c1, core, chained):
     s = stats(u, ref)
     flush_print(f"[fp32c whole wake {core} chained={chained}] {s}")
     assert s["max_rel"] <= 1e-6 and s["rms_rel"] <= 1e-7 and s["normwise"] <= 1e-7
+    # the same call again gives the same bits (fixed summation orders,
+    # near-pair sums added in list order)
+    u2 = layout.uind_to_cm(ev.eval(tg, sg, o).cpu().numpy(), c1.domain.shape)
+    assert np.array_equal(u, u2)
     # segment ranges that start inside a filament (the kernel then forms r1
     # from the range's first start point instead of carrying it)
     for split in (2, 5, 7):
