@@ -494,6 +494,18 @@ def main():
     ms_c = c0.elapsed_time(c1) / nc
     rate_c = inter / (ms_c * 1e-3) / 1e9
 
+    # the drop-in call itself: host arrays in MATLAB layout, the same segments
+    # and targets, Vatistas-2 with BiotSavart.m semantics, transfers included
+    gam, rcs = recs[:, 6].copy(), recs[:, 7].copy()
+    call = (F1, F2, wake.domain, gam, rcs, 2, "visc", 1e300, "fulc", "true", (256, 1, 0))
+    winds.BiotSavartMex(*call)
+    na = 3
+    a0 = time.perf_counter()
+    for _ in range(na):
+        winds.BiotSavartMex(*call)
+    abi_ms = (time.perf_counter() - a0) / na * 1e3
+    log(f"drop-in BiotSavartMex call with host arrays: {abi_ms:.1f} ms (transfers included)")
+
     if args.calib_copy:
         src = torch.ones(2 << 30, dtype=torch.uint8, device=dev)
         dst = torch.empty_like(src)
@@ -534,6 +546,7 @@ def main():
                        "ms_per_step": round(ms_c, 4),
                        "tflops": round(inter * FLOP_CUTOFF / (ms_c * 1e-3) / 1e12, 2)},
         "fp64_ms_per_step": round(t64 * 1e3, 2),
+        "drop_in_call_ms_host_arrays": round(abi_ms, 2),
         "batch_64_wakes": batch,
         "far_wake_surrogate": surr_rep,
         "free_wake_march": march,
