@@ -64,7 +64,8 @@ typedef struct winds_bs_opts {
     double near_len_frac;  /*   max(near_factor*r_c, near_len_frac*|L|) of   */
                            /*   the segment; <= 0 selects 40 and 0.51;       */
                            /*   fractions below 0.51 act as 0.51             */
-    int32_t block;         /* threads per block; 0 = tuned                   */
+    int32_t block;         /* threads per block; 0 = tuned; the FP32C and    */
+                           /*   MEX-semantics FP64 kernels use at most 256   */
     int32_t split;         /* segment-range split; FP32C: 0 = tuned; FP64:  */
                            /*   0/1 = one sequential sum (reference order),  */
                            /*   > 1 = split sums (for few targets)           */
