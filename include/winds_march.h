@@ -44,7 +44,7 @@ typedef struct winds_march_cfg {
     int32_t nb;            /* blades                                           */
     int32_t nst;           /* trailing nodes per blade (ns = nst-1 stations)   */
     int32_t max_ages;      /* wake truncation (>= 3)                           */
-    int32_t n_alpha;       /* polar table length                               */
+    int32_t n_alpha;       /* polar table length, 2..1024                      */
     const double *r_trail;      /* [nst] node radii (blade.RTrail)             */
     const double *chord_trail;  /* [nst] (blade.ChordTrail)                    */
     const double *twist_trail;  /* [nst] degrees (blade.AeroTwstTrail)         */

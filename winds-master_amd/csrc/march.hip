@@ -137,6 +137,31 @@ __device__ double interp1(const double *xp, const double *fp, int n, double x) {
     return slope * (x - xp[lo]) + fp[lo];
 }
 
+// interp1 of two tables over the same abscissae, with one search
+__device__ __forceinline__ void interp1x2(const double *xp, const double *fa, const double *fb,
+                                          int n, double x, double &ya, double &yb) {
+    if (!(x >= xp[0] && x <= xp[n - 1])) {
+        ya = yb = __builtin_nan("");
+        return;
+    }
+    if (x == xp[n - 1]) {
+        ya = fa[n - 1];
+        yb = fb[n - 1];
+        return;
+    }
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (xp[mid] <= x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const double dx = xp[lo + 1] - xp[lo];
+    ya = (fa[lo + 1] - fa[lo]) / dx * (x - xp[lo]) + fa[lo];
+    yb = (fb[lo + 1] - fb[lo]) / dx * (x - xp[lo]) + fb[lo];
+}
+
 // initials.m:110-125: bound circulation 0.5*|U|*chord*cl(geometric aoa)
 __global__ void k_init_bound(Geom g, double t, double umag, int A, double *__restrict__ gs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -513,6 +538,7 @@ constexpr int kKjThreads = 1024;
 constexpr int kKjWaves = kKjThreads / 64;
 constexpr int kMaxLL = 512;      // lifting-line points handled by one workgroup
 constexpr int kMaxNodesBlade = 256;
+constexpr int kMaxAlpha = 1024;  // polar table entries held in LDS by the KJ kernel
 
 // Kutta-Joukowski fixed point (KuttaJoukowski.m:42-49 with kj() :204-296)
 // for time t, in FP64, in one workgroup, followed by perform.m:42-62.
@@ -539,11 +565,36 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
     __shared__ double s_u[3 * kMaxLL];  // induced velocity at the lifting line
     __shared__ double s_red[kKjThreads];
     __shared__ double s_red2[kKjThreads];
+    __shared__ double s_al[kMaxAlpha], s_tcl[kMaxAlpha], s_tcd[kMaxAlpha];  // the polar
     __shared__ int s_nan;
     const int tid = threadIdx.x;
     for (int i = tid; i < NLL; i += kKjThreads) {
         const int m = i % ns, b = i / ns;
         s_g[i] = gs[m + (int64_t)b * ns * A];
+    }
+    for (int i = tid; i < g.n_alpha; i += kKjThreads) {
+        s_al[i] = g.alpha[i];
+        s_tcl[i] = g.cl[i];
+        s_tcd[i] = g.cd[i];
+    }
+    // Thread i < NLL owns lifting-line point i (NLL <= kMaxLL < kKjThreads).
+    // Its frame, inflow and chord do not change during the iteration.
+    double bx0 = 0.0, bx1 = 0.0, bx2 = 0.0, by0 = 0.0, by1 = 0.0, by2 = 0.0;
+    double vb0 = 0.0, vb1 = 0.0, Vinf = 0.0, chord = 0.0;
+    if (tid < NLL) {
+        const int m = tid % ns, b = tid / ns;
+        const LL L = ll_frame(g, m, b, t);
+        bx0 = L.bx[0];
+        bx1 = L.bx[1];
+        bx2 = L.bx[2];
+        by0 = L.by[0];
+        by1 = L.by[1];
+        by2 = L.by[2];
+        vb0 = L.vblade[0];
+        vb1 = L.vblade[1];
+        Vinf = sqrt(L.vblade[0] * L.vblade[0] + L.vblade[1] * L.vblade[1] +
+                    L.vblade[2] * L.vblade[2]);
+        chord = g.chord_node[m];
     }
     if (tid < nb) {
         const int b = tid;
@@ -616,40 +667,33 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
             }
         }
         __syncthreads();
-        double gnew_i[(kMaxLL + kKjThreads - 1) / kKjThreads];
+        double gnew = 0.0;
         double local_max = 0.0;
-        int q = 0;
-        for (int i = tid; i < NLL; i += kKjThreads, ++q) {
-            const int m = i % ns, b = i / ns;
+        if (tid < NLL) {
+            const int i = tid;
             const double ux = s_u[i], uy = s_u[NLL + i], uz = s_u[2 * NLL + i];
-            const LL L = ll_frame(g, m, b, t);
-            const double rx = L.bx[0] * ux + L.bx[1] * uy + L.bx[2] * uz;
-            const double ry = L.by[0] * ux + L.by[1] * uy + L.by[2] * uz;
-            const double tu = L.vblade[0] + rx, tv = L.vblade[1] + ry;
-            const double Vinf = sqrt(L.vblade[0] * L.vblade[0] + L.vblade[1] * L.vblade[1] +
-                                     L.vblade[2] * L.vblade[2]);
+            const double rx = bx0 * ux + bx1 * uy + bx2 * uz;
+            const double ry = by0 * ux + by1 * uy + by2 * uz;
+            const double tu = vb0 + rx, tv = vb1 + ry;
             const double aoa = atan2(-tv, tu) * (180.0 / kPi);
-            const double cl = interp1(g.alpha, g.cl, g.n_alpha, aoa);
-            const double cd = interp1(g.alpha, g.cd, g.n_alpha, aoa);
+            double cl, cd;
+            interp1x2(s_al, s_tcl, s_tcd, g.n_alpha, aoa, cl, cd);
             if (cl != cl) s_nan = 1;
-            const double gn = 0.5 * Vinf * g.chord_node[m] * cl;
+            const double gn = 0.5 * Vinf * chord * cl;
             const double dg = gn - s_g[i];
-            gnew_i[q] = s_g[i] + relax * dg;
-            const double dgn = fabs(dg / (fabs(gn) + 1.0));
-            local_max = fmax(local_max, dgn);
+            gnew = s_g[i] + relax * dg;
+            local_max = fabs(dg / (fabs(gn) + 1.0));
             cl_out[i] = cl;  // the last iteration's values are the step's result
             cd_out[i] = cd;
             aoa_out[i] = aoa;
         }
-        s_red[tid] = local_max;
+        // largest relative change: wave maxima, then the 16 of them
+        for (int off = 32; off > 0; off >>= 1) local_max = fmax(local_max, __shfl_xor(local_max, off));
+        if ((tid & 63) == 0) s_red[tid >> 6] = local_max;
         __syncthreads();
-        q = 0;
-        for (int i = tid; i < NLL; i += kKjThreads, ++q) s_g[i] = gnew_i[q];
-        for (int w = kKjThreads / 2; w > 0; w >>= 1) {
-            if (tid < w) s_red[tid] = fmax(s_red[tid], s_red[tid + w]);
-            __syncthreads();
-        }
+        if (tid < NLL) s_g[tid] = gnew;
         dgmax = s_red[0];
+        for (int w = 1; w < kKjWaves; ++w) dgmax = fmax(dgmax, s_red[w]);
         if (s_nan) dgmax = 0.0;  // diverging: stop, the host reports it
         ++j;
         __syncthreads();
@@ -1216,7 +1260,8 @@ int winds_march_create(const winds_march_cfg *cfg, winds_march **out) {
     if (!cfg || !out) return set_error(WINDS_E_ARG, "null argument");
     *out = nullptr;
     const winds_march_cfg &c = *cfg;
-    if (c.nb < 1 || c.nst < 2 || c.max_ages < 3 || c.n_alpha < 2 || c.max_steps < 1)
+    if (c.nb < 1 || c.nst < 2 || c.max_ages < 3 || c.n_alpha < 2 || c.n_alpha > kMaxAlpha ||
+        c.max_steps < 1)
         return set_error(WINDS_E_DIMS, "bad march sizes");
     if ((int64_t)(c.nst - 1) * c.nb > kMaxLL || c.nst > kMaxNodesBlade)
         return set_error(WINDS_E_DIMS, "too many lifting-line points for the KJ solver");
