@@ -566,6 +566,8 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
     __shared__ double s_red[kKjThreads];
     __shared__ double s_red2[kKjThreads];
     __shared__ double s_al[kMaxAlpha], s_tcl[kMaxAlpha], s_tcd[kMaxAlpha];  // the polar
+    __shared__ double s_fz[3 * kMaxLL];   // the frozen filaments' velocity at the points
+    __shared__ double s_ll[10 * kMaxLL];  // per point: chord frame, inflow, |inflow|, chord
     __shared__ int s_nan;
     const int tid = threadIdx.x;
     for (int i = tid; i < NLL; i += kKjThreads) {
@@ -578,23 +580,27 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
         s_tcd[i] = g.cd[i];
     }
     // Thread i < NLL owns lifting-line point i (NLL <= kMaxLL < kKjThreads).
-    // Its frame, inflow and chord do not change during the iteration.
-    double bx0 = 0.0, bx1 = 0.0, bx2 = 0.0, by0 = 0.0, by1 = 0.0, by2 = 0.0;
-    double vb0 = 0.0, vb1 = 0.0, Vinf = 0.0, chord = 0.0;
+    // Its frame, inflow, chord and frozen velocity do not change during the
+    // iteration: computed once, kept in LDS (in registers they pushed the
+    // kernel past its 128 VGPRs into spills).
     if (tid < NLL) {
         const int m = tid % ns, b = tid / ns;
         const LL L = ll_frame(g, m, b, t);
-        bx0 = L.bx[0];
-        bx1 = L.bx[1];
-        bx2 = L.bx[2];
-        by0 = L.by[0];
-        by1 = L.by[1];
-        by2 = L.by[2];
-        vb0 = L.vblade[0];
-        vb1 = L.vblade[1];
-        Vinf = sqrt(L.vblade[0] * L.vblade[0] + L.vblade[1] * L.vblade[1] +
+        double *q = s_ll + 10 * tid;
+        q[0] = L.bx[0];
+        q[1] = L.bx[1];
+        q[2] = L.bx[2];
+        q[3] = L.by[0];
+        q[4] = L.by[1];
+        q[5] = L.by[2];
+        q[6] = L.vblade[0];
+        q[7] = L.vblade[1];
+        q[8] = sqrt(L.vblade[0] * L.vblade[0] + L.vblade[1] * L.vblade[1] +
                     L.vblade[2] * L.vblade[2]);
-        chord = g.chord_node[m];
+        q[9] = g.chord_node[m];
+        s_fz[tid] = frozen[tid];
+        s_fz[NLL + tid] = frozen[NLL + tid];
+        s_fz[2 * NLL + tid] = frozen[2 * NLL + tid];
     }
     if (tid < nb) {
         const int b = tid;
@@ -660,9 +666,9 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
                     uz += __shfl_down(uz, off);
                 }
                 if (lane == 0) {
-                    s_u[i] = frozen[i] + ux;
-                    s_u[NLL + i] = frozen[NLL + i] + uy;
-                    s_u[2 * NLL + i] = frozen[2 * NLL + i] + uz;
+                    s_u[i] = s_fz[i] + ux;
+                    s_u[NLL + i] = s_fz[NLL + i] + uy;
+                    s_u[2 * NLL + i] = s_fz[2 * NLL + i] + uz;
                 }
             }
         }
@@ -671,10 +677,12 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
         double local_max = 0.0;
         if (tid < NLL) {
             const int i = tid;
+            const double *q = s_ll + 10 * i;
             const double ux = s_u[i], uy = s_u[NLL + i], uz = s_u[2 * NLL + i];
-            const double rx = bx0 * ux + bx1 * uy + bx2 * uz;
-            const double ry = by0 * ux + by1 * uy + by2 * uz;
-            const double tu = vb0 + rx, tv = vb1 + ry;
+            const double rx = q[0] * ux + q[1] * uy + q[2] * uz;
+            const double ry = q[3] * ux + q[4] * uy + q[5] * uz;
+            const double tu = q[6] + rx, tv = q[7] + ry;
+            const double Vinf = q[8], chord = q[9];
             const double aoa = atan2(-tv, tu) * (180.0 / kPi);
             double cl, cd;
             interp1x2(s_al, s_tcl, s_tcd, g.n_alpha, aoa, cl, cd);
