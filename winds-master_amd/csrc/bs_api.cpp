@@ -169,7 +169,10 @@ int to_cfg(const winds_bs_opts *o, winds::EvalCfg &c) {
 int pick_split(int64_t nt, int64_t nseg, int block) {
     const int64_t waves = ((nt + 63) / 64);
     const int64_t tiles = winds::seg32_padded(nseg) / winds::kSegTile;
-    const int64_t slots = (int64_t)cu_count() * 4 * 4;  // SIMDs x resident waves
+    // SIMDs x resident waves (the kernel runs 7 waves per SIMD; sized for 4,
+    // the surrogate's near-wake sum, 6,696 targets x 26,259 segments, took
+    // 324 us at 39 splits against 243 us at 68)
+    const int64_t slots = (int64_t)cu_count() * 4 * 7;
     (void)block;
     int best = 1;
     double best_cost = 1e300;
