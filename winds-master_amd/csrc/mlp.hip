@@ -89,12 +89,27 @@ __global__ __launch_bounds__(256) void gemm_nt(const bf16 *__restrict__ A, int l
     }
 }
 
+// Row and column of flat index i in rows of N: 32-bit division whenever i
+// fits (every batch these tables hold; the 64-bit form costs ~4x the
+// instructions in these memory-bound elementwise kernels).
+__device__ __forceinline__ void row_col(int64_t i, int N, int &r, int &c) {
+    if (i < (int64_t)0xffffffff) {
+        const uint32_t u = (uint32_t)i;
+        r = (int)(u / (uint32_t)N);
+        c = (int)(u - (uint32_t)r * (uint32_t)N);
+    } else {
+        r = (int)(i / N);
+        c = (int)(i % N);
+    }
+}
+
 // FP32 rows [B][n] -> BF16 [Bp][Kp] and [Kp][Bp], zero padded
 __global__ void pack_input(const float *__restrict__ X, int B, int n, int Bp, int Kp,
                            bf16 *__restrict__ Xb, bf16 *__restrict__ XT) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)Bp * Kp) return;
-    const int b = (int)(i / Kp), k = (int)(i % Kp);
+    int b, k;
+    row_col(i, Kp, b, k);
     const float x = (b < B && k < n) ? X[(int64_t)b * n + k] : 0.0f;
     const bf16 h = to_bf16(x);
     Xb[i] = h;
@@ -108,7 +123,8 @@ __global__ void bias_act(float *__restrict__ Z, const float *__restrict__ bias, 
                          int silu, bf16 *__restrict__ Ab, bf16 *__restrict__ AT) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)Bp * N) return;
-    const int b = (int)(i / N), n = (int)(i % N);
+    int b, n;
+    row_col(i, N, b, n);
     const float z = Z[i] + bias[n];
     Z[i] = z;
     if (silu) {
@@ -124,7 +140,8 @@ __global__ void mse_grad(const float *__restrict__ Y, const float *__restrict__ 
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     float contrib = 0.0f;
     if (i < (int64_t)Bp * Np) {
-        const int b = (int)(i / Np), c = (int)(i % Np);
+        int b, c;
+        row_col(i, Np, b, c);
         float g = 0.0f;
         if (b < B && c < nout) {
             const float r = Y[i] - T[(int64_t)b * nout + c];
@@ -144,7 +161,8 @@ __global__ void pack_delta(const float *__restrict__ dZ, int Bp, int N, bf16 *__
                            bf16 *__restrict__ DT) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)Bp * N) return;
-    const int b = (int)(i / N), n = (int)(i % N);
+    int b, n;
+    row_col(i, N, b, n);
     const bf16 h = to_bf16(dZ[i]);
     Db[i] = h;
     DT[(int64_t)n * Bp + b] = h;
@@ -203,7 +221,8 @@ __global__ void take_cols(const float *__restrict__ Y, int Np, int B, int nout,
                           float *__restrict__ out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)B * nout) return;
-    const int b = (int)(i / nout), c = (int)(i % nout);
+    int b, c;
+    row_col(i, nout, b, c);
     out[i] = Y[(int64_t)b * Np + c];
 }
 
