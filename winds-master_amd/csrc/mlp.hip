@@ -151,9 +151,18 @@ __global__ void mse_grad(const float *__restrict__ Y, const float *__restrict__ 
         }
         dZ[i] = g;
     }
-    // wave-level sum, one atomic per wave
+    if (!loss) return;
+    // workgroup sum, one atomic per workgroup (one per wave contended on the
+    // single address: ~28 us per step at batch 4,096)
+    __shared__ float part[16];
     for (int off = 32; off > 0; off >>= 1) contrib += __shfl_down(contrib, off);
-    if (loss && (threadIdx.x & 63) == 0 && contrib != 0.0f) atomicAdd(loss, contrib);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = contrib;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += part[w];
+        if (t != 0.0f) atomicAdd(loss, t);
+    }
 }
 
 // FP32 delta -> BF16 in both layouts

@@ -310,10 +310,13 @@ def train(net, Fn, Yn, steps: int, batch: int, lr: float = 1e-3, lr_end: float =
         xb = Fn.index_select(0, idx)
         yb = Yn.index_select(0, idx)
         r = lr_end + 0.5 * (lr - lr_end) * (1.0 + math.cos(math.pi * it / max(1, steps)))
-        net.train_step(xb, yb, r, loss)
+        # the loss (an atomic sum in the step) only where it is read
+        logged = log and (it + 1) % 1000 == 0
+        want = it >= steps - 100 or logged
+        net.train_step(xb, yb, r, loss if want else None)
         if it >= steps - 100:
             acc += loss
-        if log and (it + 1) % 1000 == 0:
+        if logged:
             log(f"  train step {it + 1}: loss {float(loss.item()):.3e}")
     torch.cuda.synchronize()
     return time.perf_counter() - t0, float(acc.item()) / min(100, steps)
