@@ -165,16 +165,51 @@ __global__ void mse_grad(const float *__restrict__ Y, const float *__restrict__ 
     }
 }
 
-// FP32 delta -> BF16 in both layouts
-__global__ void pack_delta(const float *__restrict__ dZ, int Bp, int N, bf16 *__restrict__ Db,
-                           bf16 *__restrict__ DT) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)Bp * N) return;
-    int b, n;
-    row_col(i, N, b, n);
-    const bf16 h = to_bf16(dZ[i]);
-    Db[i] = h;
-    DT[(int64_t)n * Bp + b] = h;
+// The same three conversions when the [column][row] copy is wanted too
+// (training), over 64 x 64 tiles through LDS so that both copies are
+// written with consecutive lanes on consecutive addresses (the per-element
+// kernels' transposed stores touched one 2-byte word per lane per line).
+// MODE 0: input rows X [B][n] (zero padded) -> [Bp][N] and [N][Bp];
+// MODE 1: Z += bias, A = SiLU(Z) -> [Bp][N] and [N][Bp];
+// MODE 2: delta dZ -> [Bp][N] and [N][Bp].  Bp is a multiple of 64.
+template <int MODE>
+__global__ __launch_bounds__(256) void tile_pack(float *__restrict__ Z, const float *__restrict__ X,
+                                                 const float *__restrict__ bias, int B, int n,
+                                                 int Bp, int N, bf16 *__restrict__ Ab,
+                                                 bf16 *__restrict__ AT) {
+    __shared__ bf16 tile[64][66];
+    const int t = threadIdx.x;
+    const int b0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int c = t & 63;
+    const int col = n0 + c;
+#pragma unroll 4
+    for (int k = 0; k < 16; ++k) {
+        const int r = (t >> 6) + 4 * k;
+        const int row = b0 + r;
+        if (col < N) {
+            const int64_t i = (int64_t)row * N + col;
+            float v;
+            if constexpr (MODE == 0) {
+                v = (row < B && col < n) ? X[(int64_t)row * n + col] : 0.0f;
+            } else if constexpr (MODE == 1) {
+                const float z = Z[i] + bias[col];
+                Z[i] = z;
+                v = z * sigm(z);
+            } else {
+                v = Z[i];
+            }
+            const bf16 h = to_bf16(v);
+            Ab[i] = h;
+            tile[r][c] = h;
+        }
+    }
+    __syncthreads();
+    const int rr = t & 63;
+#pragma unroll 4
+    for (int k = 0; k < 16; ++k) {
+        const int cc = (t >> 6) + 4 * k;
+        if (n0 + cc < N) AT[(int64_t)(n0 + cc) * Bp + b0 + rr] = tile[rr][cc];
+    }
 }
 
 // dZ = dA * SiLU'(Z)
@@ -296,17 +331,26 @@ void gemm(hipStream_t st, const bf16 *A, int lda, const bf16 *B, int ldb, float 
 // forward with the caches training needs (keep_t: also the [feature][batch] layouts)
 int forward(winds_mlp *m, const float *X, int B, int Bp, bool keep_t, hipStream_t st) {
     Layer &l0 = m->L[0];
-    pack_input<<<nblk((int64_t)Bp * l0.Kp), 256, 0, st>>>(X, B, m->n_in, Bp, l0.Kp, m->Xb.as<bf16>(),
-                                                          keep_t ? m->XT.as<bf16>() : nullptr);
+    if (keep_t)
+        tile_pack<0><<<dim3((l0.Kp + 63) / 64, Bp / 64), 256, 0, st>>>(
+            nullptr, X, nullptr, B, m->n_in, Bp, l0.Kp, m->Xb.as<bf16>(), m->XT.as<bf16>());
+    else
+        pack_input<<<nblk((int64_t)Bp * l0.Kp), 256, 0, st>>>(X, B, m->n_in, Bp, l0.Kp,
+                                                              m->Xb.as<bf16>(), nullptr);
     MLP_HIP(hipGetLastError());
     const bf16 *in = m->Xb.as<bf16>();
     for (int i = 0; i < m->nl; ++i) {
         Layer &l = m->L[i];
         gemm(st, in, l.Kp, l.Wb.as<bf16>(), l.Kp, l.Z.as<float>(), l.Np, Bp, l.Np, l.Kp);
         const bool hidden = i + 1 < m->nl;
-        bias_act<<<nblk((int64_t)Bp * l.Np), 256, 0, st>>>(
-            l.Z.as<float>(), l.b.as<float>(), Bp, l.Np, hidden ? 1 : 0, l.Ab.as<bf16>(),
-            (hidden && keep_t) ? l.AT.as<bf16>() : nullptr);
+        if (hidden && keep_t)
+            tile_pack<1><<<dim3((l.Np + 63) / 64, Bp / 64), 256, 0, st>>>(
+                l.Z.as<float>(), nullptr, l.b.as<float>(), Bp, l.Np, Bp, l.Np, l.Ab.as<bf16>(),
+                l.AT.as<bf16>());
+        else
+            bias_act<<<nblk((int64_t)Bp * l.Np), 256, 0, st>>>(
+                l.Z.as<float>(), l.b.as<float>(), Bp, l.Np, hidden ? 1 : 0, l.Ab.as<bf16>(),
+                nullptr);
         MLP_HIP(hipGetLastError());
         in = l.Ab.as<bf16>();
     }
@@ -320,8 +364,9 @@ int backward(winds_mlp *m, const float *T, int B, int Bp, hipStream_t st, float 
     MLP_HIP(hipGetLastError());
     for (int i = m->nl - 1; i >= 0; --i) {
         Layer &l = m->L[i];
-        pack_delta<<<nblk((int64_t)Bp * l.Np), 256, 0, st>>>(l.dZ.as<float>(), Bp, l.Np,
-                                                            l.Db.as<bf16>(), l.DT.as<bf16>());
+        tile_pack<2><<<dim3((l.Np + 63) / 64, Bp / 64), 256, 0, st>>>(
+            l.dZ.as<float>(), nullptr, nullptr, Bp, l.Np, Bp, l.Np, l.Db.as<bf16>(),
+            l.DT.as<bf16>());
         MLP_HIP(hipGetLastError());
         const bf16 *xt = i == 0 ? m->XT.as<bf16>() : m->L[i - 1].AT.as<bf16>();
         // dW [Np][Kp] = dZ^T [Np][Bp] * X^T [Kp][Bp]^T
