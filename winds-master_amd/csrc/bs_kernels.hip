@@ -13,13 +13,14 @@
 //                    268-282); bitwise equal to the CPU restatement.
 //   bs_fp64_matlab   FP64, BiotSavart.m arithmetic (:86-115) and its nested
 //                    sum order sum(sum(sum(.,1),3),4).
-//   bs_fp32c         FP32 pair arithmetic with r1 = P-A taken in FP64 and
-//                    rounded once, r2 = r1 - L, r1 x r2 evaluated as L x r1 (no
-//                    cancellation), per-tile partial sums merged into a
-//                    (hi, lo) two-sum accumulator; pairs whose target lies
-//                    within the near radius of the segment are evaluated in
-//                    FP64 with the reference arithmetic instead (inline
-//                    divergent branch, taken by few waves).
+//   bs_fp32c         FP32 pair arithmetic with r = P-B taken in FP64 and
+//                    rounded once per filament point and carried to the next
+//                    segment of the filament as its r1, r1 x r2 evaluated as
+//                    L x r1 (no cancellation), per-tile partial sums merged
+//                    into a (hi, lo) two-sum accumulator; pairs whose target
+//                    lies within the near radius of the segment are listed
+//                    per wave in LDS and evaluated in FP64 with the reference
+//                    arithmetic, 64 at a time.
 //   bs_reduce_split  sums the per-split partials in a fixed order.
 //   pack/unpack      column-major MATLAB arrays <-> device layout, in the
 //                    reference's packed index k = m + n*ns + p*ns*nt.
