@@ -65,7 +65,8 @@ __global__ void prep_seg32_batch(const Seg64 *__restrict__ seg, const BatchWake 
     const int64_t j = k - wk[w].p_off;
     const Seg64 *src = j < wk[w].ns ? seg + wk[w].s_off + j : nullptr;
     const Seg64 *prev = j > 0 && j < wk[w].ns ? src - 1 : nullptr;
-    out[k] = make_seg32(src, prev, core_kind, n, d, near_factor, near_len_frac);
+    const Seg64 *next = j + 1 < wk[w].ns ? src + 1 : nullptr;
+    out[k] = make_seg32(src, prev, next, core_kind, n, d, near_factor, near_len_frac);
 }
 
 __global__ void reset_counter(int *c) {

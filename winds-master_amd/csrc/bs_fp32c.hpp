@@ -34,11 +34,28 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Prepared record from FP64 record s (prev: the record before it in the same
-// set, or null), or a padding record when s is null.
-__device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, const Seg64 *prev, int core_kind,
-                                            int n, double d, double near_factor,
-                                            double near_len_frac) {
+// Squared near radius of FP64 record s with |L|^2 = l2: max(near_factor^2
+// rc, near_len_frac^2 |L|^2) (near_len_frac at least kMinNearLenFrac); +inf
+// (always FP64) when it is not positive or gamma is NaN.
+__device__ __forceinline__ double near_radius2(const Seg64 &s, double l2, double near_factor,
+                                               double near_len_frac) {
+    double nr2 = near_factor * near_factor * s.rc;
+    const double nlf = near_len_frac > kMinNearLenFrac ? near_len_frac : kMinNearLenFrac;
+    const double nl2 = nlf * nlf * l2;
+    if (nl2 > nr2) nr2 = nl2;
+    if (!(nr2 > 0.0) || !(s.g == s.g)) nr2 = __builtin_inf();
+    return nr2;
+}
+
+__device__ __forceinline__ bool seg_chained(const Seg64 *prev, const Seg64 &s) {
+    return prev && prev->bx == s.ax && prev->by == s.ay && prev->bz == s.az;
+}
+
+// Prepared record from FP64 record s (prev / next: the records before and
+// after it in the same set, or null), or a padding record when s is null.
+__device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, const Seg64 *prev,
+                                            const Seg64 *next, int core_kind, int n, double d,
+                                            double near_factor, double near_len_frac) {
     Seg32 r;
     if (!s_ptr) {
         r.bx = r.by = r.bz = 1.0e6;
@@ -49,7 +66,7 @@ __device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, const Seg64 *pre
         r.core = 1.0f;
         r.near2 = -1.0f;
         r.flags = kSegSkip;
-        r.unused = 0;
+        r.near2a = -1.0f;
         return r;
     }
     const Seg64 s = *s_ptr;
@@ -70,15 +87,23 @@ __device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, const Seg64 *pre
         const float c = (float)ipow(s.rc, n);
         r.core = c > FLT_MIN ? c : FLT_MIN;
     }
-    double nr2 = near_factor * near_factor * s.rc;
-    const double nlf = near_len_frac > kMinNearLenFrac ? near_len_frac : kMinNearLenFrac;
-    const double nl2 = nlf * nlf * l2;
-    if (nl2 > nr2) nr2 = nl2;
-    if (!(nr2 > 0.0) || !(s.g == s.g)) nr2 = __builtin_inf();
-    r.near2 = (float)nr2;
-    r.near2b = (float)(nr2 + l2);
-    r.flags = (prev && prev->bx == s.ax && prev->by == s.ay && prev->bz == s.az) ? kSegChained : 0;
-    r.unused = 0;
+    const double nr2 = near_radius2(s, l2, near_factor, near_len_frac);
+    // the cut-off core needs no h^2 = |r1 x r2|^2 / |L|^2 but the near test:
+    // it compares |r1 x r2|^2 with R^2 |L|^2 (a zero-length segment: with
+    // +inf, as h^2 = 0 < R^2 would)
+    r.near2 = (float)(core_kind == 0 && l2 > 0.0 ? nr2 * l2 : nr2);
+    if (core_kind == 0 && !(l2 > 0.0)) r.near2 = __builtin_inff();
+    double thr = nr2 + l2;
+    r.near2a = (float)thr;
+    if (next && seg_chained(s_ptr, *next)) {
+        const Seg64 q = *next;
+        const double qx = q.bx - q.ax, qy = q.by - q.ay, qz = q.bz - q.az;
+        const double q2 = qx * qx + qy * qy + qz * qz;
+        const double tq = near_radius2(q, q2, near_factor, near_len_frac) + q2;
+        if (tq > thr) thr = tq;
+    }
+    r.near2b = (float)thr;
+    r.flags = seg_chained(prev, s) ? kSegChained : 0;
     return r;
 }
 
@@ -234,9 +259,21 @@ template <int CORE, int SEM, bool CO>
 __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz, const Seg32 &s,
                                                  const RecF &f, const Seg64 *__restrict__ q64,
                                                  int64_t k, RelPoint a, const Fp32Params &P,
-                                                 float &qx, float &qy, float &qz,
+                                                 float &qx, float &qy, float &qz, uint64_t &mc,
                                                  NearList<CORE, SEM> &nl) {
-    if (!(s.flags & kSegChained)) a = rel_point(px, py, pz, q64->ax, q64->ay, q64->az);
+    // mc: the wave's lanes passing the end-point test at A.  A chained record
+    // takes it from the record before (whose B threshold covers this
+    // record's), the others test their own start point.  Kept as a lane mask
+    // in SGPRs: a carried per-lane bool is rebuilt from a VGPR each record.
+    // The general Vatistas order, whose prefetched records already fill the
+    // SGPRs, tests A on every record instead, against the B threshold (at
+    // least A's own): the carried mask and a further threshold made it spill
+    // SGPRs to VGPR lanes, 15% slower.
+    constexpr bool kCarry = CORE != 3;
+    if (!(s.flags & kSegChained)) {
+        a = rel_point(px, py, pz, q64->ax, q64->ay, q64->az);
+        if constexpr (kCarry) mc = __builtin_amdgcn_ballot_w64(!(a.s >= s.near2a));
+    }
     const RelPoint b = rel_point(px, py, pz, s.bx, s.by, s.bz);
     const float Lx = f.lx, Ly = f.ly, Lz = f.lz;
     const float dot = fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x));
@@ -245,14 +282,31 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     const float cy = fmaf(Lz, a.x, -(Lx * a.z));
     const float cz = fmaf(Lx, a.y, -(Ly * a.x));
     const float c2 = fmaf(cz, cz, fmaf(cy, cy, cx * cx));  // |r1 x r2|^2
-    const float h2 = c2 * f.il2;  // squared distance to the segment's line
+    // squared distance to the segment's line (the cut-off core only tests
+    // it, as c2 against R^2 |L|^2)
+    const float h2 = CORE == 0 ? c2 : c2 * f.il2;
     // Conservative near test: every target closer than R to the closed
-    // segment has h2 < R^2 and min(r1^2, r2^2) < R^2 + |L|^2.  NaN anywhere
-    // selects the FP64 branch.  R >= 0.51|L| makes every target inside the
-    // sphere on the segment as diameter (the pairs with r1.r2 <= 0) near, so
-    // below r1.r2 > 0 and r1 r2 + r1.r2 never cancels.
-    const bool n1 = !(h2 >= f.near2), n2 = !(a.s >= f.near2b), n3 = !(b.s >= f.near2b);
-    const bool near = n1 & (n2 | n3);
+    // segment has h2 < R^2 and min(r1^2, r2^2) < R^2 + |L|^2 (or a larger
+    // end-point threshold).  NaN anywhere selects the FP64 branch.
+    // R >= 0.51|L| makes every target inside the sphere on the segment as
+    // diameter (the pairs with r1.r2 <= 0) near, so below r1.r2 > 0 and
+    // r1 r2 + r1.r2 never cancels.
+    const bool n1 = !(h2 >= f.near2), n3 = !(b.s >= f.near2b);
+    // the near mask combined from compare masks in SGPRs (a ballot of a
+    // combined per-lane condition would go through a VGPR and a compare)
+    // (the general order forms its mask after the sum, where fewer SGPRs
+    // are live)
+    uint64_t m = 0;
+    bool near;
+    const bool n2 = !(a.s >= f.near2b);
+    if constexpr (kCarry) {
+        const uint64_t m3 = __builtin_amdgcn_ballot_w64(n3);
+        m = __builtin_amdgcn_ballot_w64(n1) & (mc | m3);
+        near = __builtin_amdgcn_inverse_ballot_w64(m);
+        mc = m3;
+    } else {
+        near = n1 & (n2 | n3);
+    }
     const float pp = a.i * b.i;
     const float qa = fmaf(a.s * b.s, pp, dot);  // r1 r2 + r1.r2
     const float gn = f.g * (a.i + b.i);
@@ -288,10 +342,9 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     qx = fmaf(ubar, cx, qx);
     qy = fmaf(ubar, cy, qy);
     qz = fmaf(ubar, cz, qz);
-    // the wave's near mask from the three compare masks (a ballot of the
-    // combined condition would go through a VGPR and a fourth compare)
-    const uint64_t m = __builtin_amdgcn_ballot_w64(n1) &
-                       (__builtin_amdgcn_ballot_w64(n2) | __builtin_amdgcn_ballot_w64(n3));
+    if constexpr (!kCarry)
+        m = __builtin_amdgcn_ballot_w64(n1) &
+            (__builtin_amdgcn_ballot_w64(n2) | __builtin_amdgcn_ballot_w64(n3));
     if (m) nl.add(m, near, k);
     return b;
 }
@@ -304,7 +357,8 @@ __device__ __forceinline__ Seg32 load_record(const Seg32 *__restrict__ s32, int6
     Seg32 s = s32[k];
     asm volatile("" : "+s"(s.flags)
                  : "s"(s.bx), "s"(s.by), "s"(s.bz), "s"(s.lx), "s"(s.ly), "s"(s.lz),
-                   "s"(s.near2b), "s"(s.il2), "s"(s.g), "s"(s.core), "s"(s.near2));
+                   "s"(s.near2b), "s"(s.il2), "s"(s.g), "s"(s.core), "s"(s.near2),
+                   "s"(s.near2a));
     return s;
 }
 
@@ -330,8 +384,11 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
     // P - (end point of the record before), carried along the filament; the
     // range's first record forms it from its own start point
     RelPoint e{0.f, 0.f, 0.f, 1.f, 1.f};
-    if (k0 < k1 && !(s32[k0].flags & kSegSkip))
+    uint64_t mc = 0;  // lanes passing the end-point near test at that point
+    if (k0 < k1 && !(s32[k0].flags & kSegSkip)) {
         e = rel_point(px, py, pz, s64[k0 + shift].ax, s64[k0 + shift].ay, s64[k0 + shift].az);
+        mc = __builtin_amdgcn_ballot_w64(!(e.s >= s32[k0].near2a));
+    }
     // General Vatistas order (4 waves per SIMD): the two records of the next
     // iteration are loaded while this one's first record is evaluated, so the
     // scalar-cache latency is not waited out at the top of every iteration
@@ -363,7 +420,7 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
             RelPoint f = e;
             if (!(sa.flags & kSegSkip))
                 f = fp32c_record<CORE, SEM, CO>(px, py, pz, sa, rec_fields_1(sa), s64 + k + shift,
-                                                k, e, P, qx, qy, qz, nl);
+                                                k, e, P, qx, qy, qz, mc, nl);
             if constexpr (kPrefetch) {
                 if (k + 2 < k1) {
                     ca = s32[k + 2];
@@ -374,7 +431,8 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
             }
             if (!(sb.flags & kSegSkip))
                 e = fp32c_record<CORE, SEM, CO>(px, py, pz, sb, rec_fields_1(sb),
-                                                s64 + k + 1 + shift, k + 1, f, P, qx, qy, qz, nl);
+                                                s64 + k + 1 + shift, k + 1, f, P, qx, qy, qz, mc,
+                                                nl);
             else
                 e = f;
             // flushed here, between record pairs, where little else is live

@@ -122,7 +122,8 @@ __global__ void prep_seg32(const Seg64 *__restrict__ seg, int64_t nseg,
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nseg_pad) return;
     out[k] = make_seg32(k < nseg ? seg + k : nullptr, k > 0 && k < nseg ? seg + k - 1 : nullptr,
-                        core_kind, n, d, near_factor, near_len_frac);
+                        k + 1 < nseg ? seg + k + 1 : nullptr, core_kind, n, d, near_factor,
+                        near_len_frac);
 }
 
 // One launch over a single segment set: grid.x covers targets, grid.y

@@ -25,13 +25,17 @@ constexpr int kSegTile = WINDS_SEG_TILE;
 struct __attribute__((aligned(64))) Seg32 {
     double bx, by, bz;   // F2 endpoint
     float lx, ly, lz;    // L = B - A, rounded once
-    float near2b;        // near2 + |L|^2 (second half of the near test)
+    float near2b;        // end-point threshold at B: near2 + |L|^2, raised to
+                         // the next record's near2 + |L|^2 when that record
+                         // is chained, so one compare at B serves both
     float il2;           // 1/|L|^2 (0 for a zero-length segment)
     float g;             // gamma / (4 pi) (0 for a zero-length segment)
     float core;          // cut-off: d |L|^2; Vatistas: max(rc^n, FLT_MIN) (rc = r_c^2)
-    float near2;         // squared near radius; +inf: always FP64, <0: never
+    float near2;         // squared near radius R^2 (cut-off core: R^2 |L|^2,
+                         // compared with |r1 x r2|^2); +inf: always FP64
     int32_t flags;       // kSegChained, kSegSkip
-    int32_t unused;
+    float near2a;        // end-point threshold at A: near2 + |L|^2 (read when
+                         // the record is not chained)
 };
 constexpr int32_t kSegChained = 1;  // starts at the previous record's end point
 constexpr int32_t kSegSkip = 2;     // padding: no term
