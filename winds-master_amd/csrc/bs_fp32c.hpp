@@ -255,7 +255,10 @@ struct NearList {
 // fields f) added to (qx, qy, qz) in FP32, or to (ax, ay, az) by the FP64
 // reference arithmetic for a near pair.  Returns P - B for the next record
 // of the filament.
-template <int CORE, int SEM, bool CO>
+// SECOND: the record is the second of a pair; when it continues the first
+// one's filament (and the core is Vatistas n = 1 or 2) it forms P - B as
+// r1 - L in FP32.
+template <int CORE, int SEM, bool CO, bool SECOND>
 __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz, const Seg32 &s,
                                                  const RecF &f, const Seg64 *__restrict__ q64,
                                                  int64_t k, RelPoint a, const Fp32Params &P,
@@ -270,12 +273,28 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     // least A's own): the carried mask and a further threshold made it spill
     // SGPRs to VGPR lanes, 15% slower.
     constexpr bool kCarry = CORE != 3;
-    if (!(s.flags & kSegChained)) {
+    const bool chained = s.flags & kSegChained;
+    if (!chained) {
         a = rel_point(px, py, pz, q64->ax, q64->ay, q64->az);
         if constexpr (kCarry) mc = __builtin_amdgcn_ballot_w64(!(a.s >= s.near2a));
     }
-    const RelPoint b = rel_point(px, py, pz, s.bx, s.by, s.bz);
     const float Lx = f.lx, Ly = f.ly, Lz = f.lz;
+    // P - B: in FP64 and rounded once, or (SECOND, chained) as r1 - L in
+    // FP32, where r1 was rounded from FP64 one record before.  The pairs that
+    // reach the FP32 arithmetic have |r2| >= 0.51|L|, so the subtraction adds
+    // a few roundings of |r2| at most.  Vatistas n = 1, 2 only: the cut-off
+    // kernel measured 4% slower with it, and in the general order the second
+    // code path made the prefetched records spill SGPRs.
+    RelPoint b;
+    if (SECOND && (CORE == 1 || CORE == 2) && chained) {
+        b.x = a.x - Lx;
+        b.y = a.y - Ly;
+        b.z = a.z - Lz;
+        b.s = fmaf(b.z, b.z, fmaf(b.y, b.y, b.x * b.x));
+        b.i = __builtin_amdgcn_rsqf(b.s);
+    } else {
+        b = rel_point(px, py, pz, s.bx, s.by, s.bz);
+    }
     const float dot = fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x));
     // r1 x r2 = L x r1 with L = B - A.
     const float cx = fmaf(Ly, a.z, -(Lz * a.y));
@@ -419,7 +438,7 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
             }
             RelPoint f = e;
             if (!(sa.flags & kSegSkip))
-                f = fp32c_record<CORE, SEM, CO>(px, py, pz, sa, rec_fields_1(sa), s64 + k + shift,
+                f = fp32c_record<CORE, SEM, CO, false>(px, py, pz, sa, rec_fields_1(sa), s64 + k + shift,
                                                 k, e, P, qx, qy, qz, mc, nl);
             if constexpr (kPrefetch) {
                 if (k + 2 < k1) {
@@ -430,9 +449,9 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                 sb = load_record(s32, k + 1);
             }
             if (!(sb.flags & kSegSkip))
-                e = fp32c_record<CORE, SEM, CO>(px, py, pz, sb, rec_fields_1(sb),
-                                                s64 + k + 1 + shift, k + 1, f, P, qx, qy, qz, mc,
-                                                nl);
+                e = fp32c_record<CORE, SEM, CO, true>(px, py, pz, sb, rec_fields_1(sb),
+                                                      s64 + k + 1 + shift, k + 1, f, P, qx, qy,
+                                                      qz, mc, nl);
             else
                 e = f;
             // flushed here, between record pairs, where little else is live
