@@ -98,7 +98,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CORE == 3 ?
         const bool active = lane < w.tn;
         const int64_t t = w.t0 + (active ? lane : 0);
         double sx, sy, sz;
-        fp32c_range<CORE, SEM, CO>(tx[t], ty[t], tz[t], s32, s64, w.shift, w.k0, w.k1, P,
+        // P - B in FP64 at every record: r1 - L in FP32 for every second
+        // record gave the batch no speed (993 G int/s both ways) and raised
+        // its max error on four wakes from 3.8e-7 to 8.7e-7
+        fp32c_range<CORE, SEM, CO, false>(tx[t], ty[t], tz[t], s32, s64, w.shift, w.k0, w.k1, P,
                                    near_lds + (lane >> 6), sx, sy, sz);
         if (active) {
             double *o = part + (int64_t)w.slot * 3 * B;

@@ -391,7 +391,9 @@ __device__ __forceinline__ Seg32 load_record(const Seg32 *__restrict__ s32, int6
 // SEM:  0 MEX semantics (near pairs by biot_p2p), 1 BiotSavart.m semantics
 //       (near pairs by the BiotSavart.m arithmetic with zeroing).
 // CO:   apply the far cut-off r1 > co && r2 > co (BiotSavart.m:111).
-template <int CORE, int SEM, bool CO>
+// PAIR32: the second record of each pair forms P - B as r1 - L in FP32
+// when it continues the first one's filament (Vatistas n = 1, 2).
+template <int CORE, int SEM, bool CO, bool PAIR32 = true>
 __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                                             const Seg32 *__restrict__ s32,
                                             const Seg64 *__restrict__ s64, int64_t shift,
@@ -449,7 +451,7 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                 sb = load_record(s32, k + 1);
             }
             if (!(sb.flags & kSegSkip))
-                e = fp32c_record<CORE, SEM, CO, true>(px, py, pz, sb, rec_fields_1(sb),
+                e = fp32c_record<CORE, SEM, CO, PAIR32>(px, py, pz, sb, rec_fields_1(sb),
                                                       s64 + k + 1 + shift, k + 1, f, P, qx, qy,
                                                       qz, mc, nl);
             else
