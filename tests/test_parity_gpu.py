@@ -279,6 +279,61 @@ def test_fp32c_targets_inside_segment_sphere_take_fp64(need_gpu, core):
             assert np.array_equal(u32, u64), (trial, frac, np.max(np.abs(u32 - u64)))
 
 
+@pytest.mark.parametrize("core", ["vatistas2", "cutoff"])
+def test_fp32c_chained_filament_near_pairs(need_gpu, core):
+    """A filament of chained segments whose lengths (and so near radii)
+    alternate by a factor of up to 60, with targets inside each segment's
+    sphere a hair off its line (where r1 r2 + r1.r2 nearly cancels and an
+    FP32 term would be wrong in its leading digits) and just beyond each end
+    point.  The near test of a chained record takes its start-point half
+    from the record before, with that record's end-point threshold raised to
+    cover both; every such pair must still take the FP64 term, so each
+    target's velocity (dominated by that term) matches FP64 to 1e-6."""
+    import torch
+    from winds.device import Evaluator
+    ev = Evaluator(0)
+    if core == "cutoff":
+        kw = dict(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, d=1e-4)
+    else:
+        kw = dict(core=winds.CORE_VATISTAS, vat_n=2, semantics=winds.SEM_MATLAB)
+    rng = np.random.default_rng(23)
+    ns = 41
+    lens = np.where(np.arange(ns) % 2 == 0, rng.uniform(0.2, 0.5, ns), rng.uniform(6.0, 12.0, ns))
+    dirs = rng.normal(size=(ns, 3)) * 0.3 + np.array([1.0, 0.2, 0.0])
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    X = np.vstack([rng.uniform(-500.0, 500.0, 3), np.zeros((ns, 3))])
+    for j in range(ns):
+        X[j + 1] = X[j] + lens[j] * dirs[j]
+    F1 = X[:-1].reshape(ns, 3, 1, 1)
+    F2 = X[1:].reshape(ns, 3, 1, 1)
+    g = rng.uniform(20.0, 60.0, ns)
+    rc = np.full(ns, 1e-8)                      # squared core radius: R = 0.51|L|
+    recs = layout.segments(F1, F2, g, rc)
+    assert np.all(recs[1:, 0:3] == recs[:-1, 3:6])   # one chained filament
+    P = []
+    for j in range(ns):
+        A, B = X[j], X[j + 1]
+        L = B - A
+        e1 = np.cross(L, [0.3, -0.7, 0.5])
+        e1 /= np.linalg.norm(e1)
+        for t in np.linspace(0.1, 0.9, 9):      # inside the sphere, off the line
+            for h in (1e-4, 1e-3, 1e-2):
+                P.append(A + t * L + h * np.linalg.norm(L) * e1)
+        for sgn, E in ((-1.0, A), (1.0, B)):    # just beyond each end point
+            P.append(E + sgn * 0.2 * L + 0.05 * np.linalg.norm(L) * e1)
+    P = np.array(P)
+    sg = torch.from_numpy(recs).cuda()
+    tg = torch.from_numpy(np.ascontiguousarray(P.T)).cuda()
+    flush_print(f"[chained near pairs {core}] {P.shape[0]} targets x {ns} segments")
+    u64 = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP64, **kw)).cpu().numpy()
+    u32 = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP32C, near_factor=1e-3,
+                                     **kw)).cpu().numpy()
+    n64 = np.linalg.norm(u64, axis=0)
+    rel = np.linalg.norm(u32 - u64, axis=0) / n64
+    assert np.all(n64 > 0)
+    assert rel.max() <= 1e-6, (rel.max(), int(rel.argmax()))
+
+
 # --------------------------------------------------------------------------
 # full single-wake size: 133,920 targets x 263,427 segments
 # --------------------------------------------------------------------------
