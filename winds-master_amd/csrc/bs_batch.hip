@@ -73,11 +73,14 @@ __global__ void reset_counter(int *c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
 }
 
-// 7 waves per SIMD (at most 72 VGPRs): the compiler's own choice, 74 VGPRs
-// and 6 waves, measured 9% slower on the 64-wake batch.  The general
-// Vatistas order keeps its 4 waves (its FP64 branch calls pow).
+// 8 waves per SIMD (at most 64 VGPRs), with the second record of each
+// pair formed in FP32 (r1 - L), which is what fits the loop in 64 VGPRs:
+// 1,071 G int/s on the 64-wake batch against 992 at 7 waves (72 VGPRs) and
+// 943 at 8 waves with FP64 end points throughout (spills).  The compiler's
+// own choice, 74 VGPRs and 6 waves, measured 9% slower than 7 waves.  The
+// general Vatistas order keeps its 4 waves (its FP64 branch calls pow).
 template <int CORE, int SEM, bool CO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CORE == 3 ? 4 : 7))) void bs_batch_persistent(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CORE == 3 ? 4 : 8))) void bs_batch_persistent(
     const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
     const Seg32 *__restrict__ s32, const Seg64 *__restrict__ s64,
     const WorkItem *__restrict__ items, int n_items, int *__restrict__ counter, Fp32Params P,
@@ -98,10 +101,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CORE == 3 ?
         const bool active = lane < w.tn;
         const int64_t t = w.t0 + (active ? lane : 0);
         double sx, sy, sz;
-        // P - B in FP64 at every record: r1 - L in FP32 for every second
-        // record gave the batch no speed (993 G int/s both ways) and raised
-        // its max error on four wakes from 3.8e-7 to 8.7e-7
-        fp32c_range<CORE, SEM, CO, false>(tx[t], ty[t], tz[t], s32, s64, w.shift, w.k0, w.k1, P,
+        fp32c_range<CORE, SEM, CO>(tx[t], ty[t], tz[t], s32, s64, w.shift, w.k0, w.k1, P,
                                    near_lds + (lane >> 6), sx, sy, sz);
         if (active) {
             double *o = part + (int64_t)w.slot * 3 * B;

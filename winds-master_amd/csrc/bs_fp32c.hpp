@@ -256,8 +256,8 @@ struct NearList {
 // reference arithmetic for a near pair.  Returns P - B for the next record
 // of the filament.
 // SECOND: the record is the second of a pair; when it continues the first
-// one's filament (and the core is Vatistas n = 1 or 2) it forms P - B as
-// r1 - L in FP32.
+// one's filament (and the core is not the general Vatistas order) it forms
+// P - B as r1 - L in FP32.
 template <int CORE, int SEM, bool CO, bool SECOND>
 __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz, const Seg32 &s,
                                                  const RecF &f, const Seg64 *__restrict__ q64,
@@ -282,11 +282,10 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     // P - B: in FP64 and rounded once, or (SECOND, chained) as r1 - L in
     // FP32, where r1 was rounded from FP64 one record before.  The pairs that
     // reach the FP32 arithmetic have |r2| >= 0.51|L|, so the subtraction adds
-    // a few roundings of |r2| at most.  Vatistas n = 1, 2 only: the cut-off
-    // kernel measured 4% slower with it, and in the general order the second
-    // code path made the prefetched records spill SGPRs.
+    // a few roundings of |r2| at most.  Not in the general Vatistas order:
+    // there the second code path made the prefetched records spill SGPRs.
     RelPoint b;
-    if (SECOND && (CORE == 1 || CORE == 2) && chained) {
+    if (SECOND && CORE != 3 && chained) {
         b.x = a.x - Lx;
         b.y = a.y - Ly;
         b.z = a.z - Lz;
@@ -392,7 +391,8 @@ __device__ __forceinline__ Seg32 load_record(const Seg32 *__restrict__ s32, int6
 //       (near pairs by the BiotSavart.m arithmetic with zeroing).
 // CO:   apply the far cut-off r1 > co && r2 > co (BiotSavart.m:111).
 // PAIR32: the second record of each pair forms P - B as r1 - L in FP32
-// when it continues the first one's filament (Vatistas n = 1, 2).
+// when it continues the first one's filament (not the general Vatistas
+// order).
 template <int CORE, int SEM, bool CO, bool PAIR32 = true>
 __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                                             const Seg32 *__restrict__ s32,
