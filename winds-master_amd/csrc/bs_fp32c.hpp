@@ -124,10 +124,10 @@ struct Fp32Params {
     double co;        // far cut-off (near pairs)
 };
 
-// FP32 view of P - X for a filament point X: the vector, its squared length
-// and the reciprocal of its length.
+// FP32 view of P - X for a filament point X: the vector, its squared length,
+// the reciprocal of its length and its length.
 struct RelPoint {
-    float x, y, z, s, i;
+    float x, y, z, s, i, r;
 };
 
 __device__ __forceinline__ RelPoint rel_point(double px, double py, double pz, double x, double y,
@@ -140,6 +140,7 @@ __device__ __forceinline__ RelPoint rel_point(double px, double py, double pz, d
     r.z = (float)(pz - z);
     r.s = fmaf(r.z, r.z, fmaf(r.y, r.y, r.x * r.x));
     r.i = __builtin_amdgcn_rsqf(r.s);
+    r.r = r.s * r.i;
     return r;
 }
 
@@ -291,6 +292,7 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
         b.z = a.z - Lz;
         b.s = fmaf(b.z, b.z, fmaf(b.y, b.y, b.x * b.x));
         b.i = __builtin_amdgcn_rsqf(b.s);
+        b.r = b.s * b.i;
     } else {
         b = rel_point(px, py, pz, s.bx, s.by, s.bz);
     }
@@ -325,8 +327,11 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     } else {
         near = n1 & (n2 | n3);
     }
+    // r1 r2 + r1.r2: from the carried lengths (one product per point, where
+    // s1 s2 / (r1 r2) costs two per record), or, for the cut-off core, whose
+    // d|L|^2 term needs 1/(r1 r2) anyway, from that
     const float pp = a.i * b.i;
-    const float qa = fmaf(a.s * b.s, pp, dot);  // r1 r2 + r1.r2
+    const float qa = CORE == 0 ? fmaf(a.s * b.s, pp, dot) : fmaf(a.r, b.r, dot);
     const float gn = f.g * (a.i + b.i);
     float ubar;
     if constexpr (CORE == 0) {
@@ -404,7 +409,7 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
     nl.init(nlds, s64, shift, k0, px, py, pz);
     // P - (end point of the record before), carried along the filament; the
     // range's first record forms it from its own start point
-    RelPoint e{0.f, 0.f, 0.f, 1.f, 1.f};
+    RelPoint e{0.f, 0.f, 0.f, 1.f, 1.f, 1.f};
     uint64_t mc = 0;  // lanes passing the end-point near test at that point
     if (k0 < k1 && !(s32[k0].flags & kSegSkip)) {
         e = rel_point(px, py, pz, s64[k0 + shift].ax, s64[k0 + shift].ay, s64[k0 + shift].az);
