@@ -166,14 +166,15 @@ int to_cfg(const winds_bs_opts *o, winds::EvalCfg &c) {
 // out the waves that take the FP64 near-pair branch; measured on the
 // 1,440-age wake, Vatistas-2: 32 splits 35.0 ms, 64-96 splits 34.0-34.1 ms,
 // 129 splits 34.6 ms, 258 splits 35.9 ms).
-int pick_split(int64_t nt, int64_t nseg, int block) {
+int pick_split(int64_t nt, int64_t nseg, const winds::EvalCfg &c) {
     const int64_t waves = ((nt + 63) / 64);
     const int64_t tiles = winds::seg32_padded(nseg) / winds::kSegTile;
-    // SIMDs x resident waves (the kernel runs 7 waves per SIMD; sized for 4,
-    // the surrogate's near-wake sum, 6,696 targets x 26,259 segments, took
-    // 324 us at 39 splits against 243 us at 68)
-    const int64_t slots = (int64_t)cu_count() * 4 * 7;
-    (void)block;
+    // SIMDs x resident waves: the kernel runs 8 waves per SIMD, 4 for the
+    // general Vatistas order (sized for 4 waves where 7 ran, the surrogate's
+    // near-wake sum, 6,696 targets x 26,259 segments, took 324 us at 39
+    // splits against 243 us at 68)
+    const bool general = c.core != 0 && c.vat_n != 1 && c.vat_n != 2;
+    const int64_t slots = (int64_t)cu_count() * 4 * (general ? 4 : 8);
     int best = 1;
     double best_cost = 1e300;
     for (int s = 1; s <= 1024; ++s) {
@@ -200,7 +201,7 @@ size_t ws_bytes(int64_t nt, int64_t nseg, const winds::EvalCfg &c, int *split_ou
         if (split_out) *split_out = split;
         return split > 1 ? align256((size_t)split * 3 * (size_t)nt * sizeof(double)) + 256 : 256;
     }
-    const int split = c.split > 0 ? c.split : pick_split(nt, nseg, c.block);
+    const int split = c.split > 0 ? c.split : pick_split(nt, nseg, c);
     if (split_out) *split_out = split;
     const size_t s32 = align256((size_t)winds::seg32_padded(nseg) * sizeof(winds::Seg32));
     const size_t part = split > 1 ? align256((size_t)split * 3 * (size_t)nt * sizeof(double)) : 0;
