@@ -198,6 +198,36 @@ def test_zero_length_segment_and_co(need_gpu, type):
         assert abs(u[0, 2, 0, 0] - ref[0, 2, 0, 0]) <= tol * abs(ref[0, 2, 0, 0])
 
 
+@pytest.mark.parametrize("type", ["fuln", "fulc"])
+def test_nan_circulation(need_gpu, type):
+    """One segment of a chained filament carries a NaN circulation.  MEX
+    semantics propagate it to every target, as biot_p2p's sum does
+    (BiotSavartMex.cu:68-92); BiotSavart.m semantics zero that segment's
+    NaN terms (BiotSavart.m:111) and the rest of the sum stands.  The FP32
+    path sends a NaN record to the FP64 arithmetic, so it behaves the same."""
+    rng = np.random.default_rng(5)
+    ns = 40
+    X = np.cumsum(rng.normal(size=(ns + 1, 3)), axis=0)
+    F1 = np.asfortranarray(X[:-1].reshape(ns, 3, 1, 1))
+    F2 = np.asfortranarray(X[1:].reshape(ns, 3, 1, 1))
+    g = rng.uniform(1.0, 5.0, ns)
+    g[17] = np.nan
+    rc = np.full(ns, 1e-3)
+    P = np.asfortranarray((X[:-1] + 0.5 * (X[1:] - X[:-1])
+                           + rng.normal(size=(ns, 3)) * 0.5).reshape(ns, 3, 1, 1))
+    u, _ = winds.BiotSavartMex(F1, F2, P, g, rc, 1e-4, "none", 1.0, type)
+    assert np.all(np.isnan(u))
+    for cmod, d, dd in (("cutm", 1e-4, 1e-4), ("visc", 2, "visc2")):
+        u, _ = winds.BiotSavartMex(F1, F2, P, g, rc, d, cmod, 1000.0, type)
+        ref = O.bs_matlab(F1, F2, P, g.reshape(ns, 1, 1, 1), rc.reshape(ns, 1, 1, 1), dd, co=1000.0)
+        assert np.isfinite(u).all(), cmod
+        s = stats(u, ref)
+        if type == "fuln":
+            assert s["normwise"] <= 1e-12, (cmod, s)
+        else:
+            assert s["max_rel"] <= 1e-6, (cmod, s)
+
+
 @pytest.mark.parametrize("threads", [64, 128, 512, 1024, 0])
 def test_block_sizes_do_not_change_fp64_bits(c1, threads):
     F1, F2, g, rc = c1.trail_segments()
