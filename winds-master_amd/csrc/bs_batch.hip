@@ -77,10 +77,9 @@ __global__ void reset_counter(int *c) {
 // pair formed in FP32 (r1 - L), which is what fits the loop in 64 VGPRs:
 // 1,071 G int/s on the 64-wake batch against 992 at 7 waves (72 VGPRs) and
 // 943 at 8 waves with FP64 end points throughout (spills).  The compiler's
-// own choice, 74 VGPRs and 6 waves, measured 9% slower than 7 waves.  The
-// general Vatistas order keeps its 4 waves (its FP64 branch calls pow).
+// own choice, 74 VGPRs and 6 waves, measured 9% slower than 7 waves.
 template <int CORE, int SEM, bool CO>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CORE == 3 ? 4 : 8))) void bs_batch_persistent(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void bs_batch_persistent(
     const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
     const Seg32 *__restrict__ s32, const Seg64 *__restrict__ s64,
     const WorkItem *__restrict__ items, int n_items, int *__restrict__ counter, Fp32Params P,

@@ -215,7 +215,7 @@ struct NearList {
                     mex_term(q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, px, py, pz, P.d, ex, ey,
                              ez);
                 } else {
-                    matlab_pair<CORE != 0, (CORE == 1 || CORE == 2) ? CORE : 0>(
+                    matlab_pair<CORE != 0, (CORE == 1 || CORE == 2) ? CORE : -1>(
                         q.ax, q.ay, q.az, q.bx, q.by, q.bz, q.g, q.rc, px, py, pz, P.d, P.vat_n,
                         P.co, ex, ey, ez);
                 }
@@ -257,8 +257,7 @@ struct NearList {
 // reference arithmetic for a near pair.  Returns P - B for the next record
 // of the filament.
 // SECOND: the record is the second of a pair; when it continues the first
-// one's filament (and the core is not the general Vatistas order) it forms
-// P - B as r1 - L in FP32.
+// one's filament it forms P - B as r1 - L in FP32.
 template <int CORE, int SEM, bool CO, bool SECOND>
 __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz, const Seg32 &s,
                                                  const RecF &f, const Seg64 *__restrict__ q64,
@@ -269,24 +268,18 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     // takes it from the record before (whose B threshold covers this
     // record's), the others test their own start point.  Kept as a lane mask
     // in SGPRs: a carried per-lane bool is rebuilt from a VGPR each record.
-    // The general Vatistas order, whose prefetched records already fill the
-    // SGPRs, tests A on every record instead, against the B threshold (at
-    // least A's own): the carried mask and a further threshold made it spill
-    // SGPRs to VGPR lanes, 15% slower.
-    constexpr bool kCarry = CORE != 3;
     const bool chained = s.flags & kSegChained;
     if (!chained) {
         a = rel_point(px, py, pz, q64->ax, q64->ay, q64->az);
-        if constexpr (kCarry) mc = __builtin_amdgcn_ballot_w64(!(a.s >= s.near2a));
+        mc = __builtin_amdgcn_ballot_w64(!(a.s >= s.near2a));
     }
     const float Lx = f.lx, Ly = f.ly, Lz = f.lz;
     // P - B: in FP64 and rounded once, or (SECOND, chained) as r1 - L in
     // FP32, where r1 was rounded from FP64 one record before.  The pairs that
     // reach the FP32 arithmetic have |r2| >= 0.51|L|, so the subtraction adds
-    // a few roundings of |r2| at most.  Not in the general Vatistas order:
-    // there the second code path made the prefetched records spill SGPRs.
+    // a few roundings of |r2| at most.
     RelPoint b;
-    if (SECOND && CORE != 3 && chained) {
+    if (SECOND && chained) {
         b.x = a.x - Lx;
         b.y = a.y - Ly;
         b.z = a.z - Lz;
@@ -314,19 +307,10 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     const bool n1 = !(h2 >= f.near2), n3 = !(b.s >= f.near2b);
     // the near mask combined from compare masks in SGPRs (a ballot of a
     // combined per-lane condition would go through a VGPR and a compare)
-    // (the general order forms its mask after the sum, where fewer SGPRs
-    // are live)
-    uint64_t m = 0;
-    bool near;
-    const bool n2 = !(a.s >= f.near2b);
-    if constexpr (kCarry) {
-        const uint64_t m3 = __builtin_amdgcn_ballot_w64(n3);
-        m = __builtin_amdgcn_ballot_w64(n1) & (mc | m3);
-        near = __builtin_amdgcn_inverse_ballot_w64(m);
-        mc = m3;
-    } else {
-        near = n1 & (n2 | n3);
-    }
+    const uint64_t m3 = __builtin_amdgcn_ballot_w64(n3);
+    const uint64_t m = __builtin_amdgcn_ballot_w64(n1) & (mc | m3);
+    const bool near = __builtin_amdgcn_inverse_ballot_w64(m);
+    mc = m3;
     // r1 r2 + r1.r2: from the carried lengths (one product per point, where
     // s1 s2 / (r1 r2) costs two per record), or, for the cut-off core, whose
     // d|L|^2 term needs 1/(r1 r2) anyway, from that
@@ -365,9 +349,6 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     qx = fmaf(ubar, cx, qx);
     qy = fmaf(ubar, cy, qy);
     qz = fmaf(ubar, cz, qz);
-    if constexpr (!kCarry)
-        m = __builtin_amdgcn_ballot_w64(n1) &
-            (__builtin_amdgcn_ballot_w64(n2) | __builtin_amdgcn_ballot_w64(n3));
     if (m) nl.add(m, near, k);
     return b;
 }
@@ -396,8 +377,7 @@ __device__ __forceinline__ Seg32 load_record(const Seg32 *__restrict__ s32, int6
 //       (near pairs by the BiotSavart.m arithmetic with zeroing).
 // CO:   apply the far cut-off r1 > co && r2 > co (BiotSavart.m:111).
 // PAIR32: the second record of each pair forms P - B as r1 - L in FP32
-// when it continues the first one's filament (not the general Vatistas
-// order).
+// when it continues the first one's filament.
 template <int CORE, int SEM, bool CO, bool PAIR32 = true>
 __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                                             const Seg32 *__restrict__ s32,
@@ -415,20 +395,10 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
         e = rel_point(px, py, pz, s64[k0 + shift].ax, s64[k0 + shift].ay, s64[k0 + shift].az);
         mc = __builtin_amdgcn_ballot_w64(!(e.s >= s32[k0].near2a));
     }
-    // General Vatistas order (4 waves per SIMD): the two records of the next
-    // iteration are loaded while this one's first record is evaluated, so the
-    // scalar-cache latency is not waited out at the top of every iteration
-    // (measured 13% faster).  At 7-8 waves per SIMD the other waves hide that
-    // latency and the early loads cost more than they save (cut-off: 6%
-    // slower), so there each iteration loads its own records.
-    constexpr bool kPrefetch = CORE == 3;
-    Seg32 ca, cb;
-    if constexpr (kPrefetch) {
-        if (k0 < k1) {
-            ca = s32[k0];
-            cb = s32[k0 + 1];
-        }
-    }
+    // Each iteration loads its own records: at 8 waves per SIMD the other
+    // waves hide the scalar-cache latency (loading the next iteration's
+    // records early helped only the general Vatistas order while it ran 4
+    // waves, and made the cut-off 6% slower).
     for (int64_t kt = k0; kt < k1; kt += kSegTile) {
         float qx = 0.f, qy = 0.f, qz = 0.f;
         // two records per iteration: the carried point alternates between
@@ -436,25 +406,12 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
 #pragma unroll 1
         for (int j = 0; j < kSegTile; j += 2) {
             const int64_t k = kt + j;
-            Seg32 sa, sb;
-            if constexpr (kPrefetch) {
-                sa = ca;
-                sb = cb;
-            } else {
-                sa = load_record(s32, k);
-            }
+            const Seg32 sa = load_record(s32, k);
             RelPoint f = e;
             if (!(sa.flags & kSegSkip))
                 f = fp32c_record<CORE, SEM, CO, false>(px, py, pz, sa, rec_fields_1(sa), s64 + k + shift,
                                                 k, e, P, qx, qy, qz, mc, nl);
-            if constexpr (kPrefetch) {
-                if (k + 2 < k1) {
-                    ca = s32[k + 2];
-                    cb = s32[k + 3];
-                }
-            } else {
-                sb = load_record(s32, k + 1);
-            }
+            const Seg32 sb = load_record(s32, k + 1);
             if (!(sb.flags & kSegSkip))
                 e = fp32c_record<CORE, SEM, CO, PAIR32>(px, py, pz, sb, rec_fields_1(sb),
                                                       s64 + k + 1 + shift, k + 1, f, P, qx, qy,

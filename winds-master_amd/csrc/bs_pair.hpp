@@ -75,12 +75,28 @@ __device__ __forceinline__ double inv_root(double y, int n) {
     return pow(y, -1.0 / (double)n);
 }
 
+// y^(-1/n) for n >= 1 without pow: an FP32 first guess and three Newton
+// steps on r^-n = y in FP64 (quadratic convergence from ~1e-6).
+__device__ __forceinline__ double inv_root_newton(double y, int n) {
+    const double inv_n = 1.0 / (double)n;
+    double r = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)y) * (float)(-inv_n));
+    for (int it = 0; it < 3; ++it) {
+        double rn = r;
+        for (int i = 1; i < n; ++i) rn = rn * r;
+        r = fma(r * fma(-y, rn, 1.0), inv_n, r);
+    }
+    return r;
+}
+
 // The same with the order fixed at compile time (NF = 1 or 2) or taken at
-// run time (NF = 0).
+// run time (NF = 0: pow, as the reference; NF = -1: Newton steps, for the
+// FP32 kernels' near pairs, where pow's registers cost the kernel half its
+// waves).
 template <int NF>
 __device__ __forceinline__ double inv_root_n(double y, int n) {
     if constexpr (NF == 1) return 1.0 / y;
     if constexpr (NF == 2) return 1.0 / sqrt(y);
+    if constexpr (NF == -1) return inv_root_newton(y, n);
     return inv_root(y, n);
 }
 
