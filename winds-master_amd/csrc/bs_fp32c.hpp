@@ -16,16 +16,6 @@ namespace winds {
 // FP32 arithmetic never meets r1 r2 + r1.r2 ~ 0 (0.5 plus rounding margin).
 constexpr double kMinNearLenFrac = 0.51;
 
-// Copy of a wave-uniform value into a VGPR.  gfx950 issues FP32 VALU
-// operations whose operands are all VGPRs at twice the rate of those that
-// read an SGPR, and a move takes one fast slot: a record value feeding
-// several products is moved once.
-__device__ __forceinline__ float to_vgpr(float s) {
-    float v;
-    asm("v_mov_b32 %0, %1" : "=v"(v) : "s"(s));
-    return v;
-}
-
 // Orders the wave's LDS accesses: LDS operations of one wave execute in
 // issue order; this keeps the compiler from moving them across each other.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -144,17 +134,16 @@ __device__ __forceinline__ RelPoint rel_point(double px, double py, double pz, d
     return r;
 }
 
-// The FP32 fields of a record as the pair arithmetic reads them: each either
-// left in its SGPR (one slow operand) or moved to a VGPR once (one fast move)
-// when it feeds more than one operation.
+// The FP32 fields of a record as the pair arithmetic reads them, all as SGPR
+// operands.  (Moving the three components of L, which feed three operations
+// each, into VGPRs first paid off at 7 waves per SIMD; at 8 waves the moves
+// cost more than the SGPR operands: 2% slower.)
 struct RecF {
     float lx, ly, lz, near2b, il2, g, core, near2;
 };
 
-// L (three uses per component) in VGPRs, the rest SGPR operands.
 __device__ __forceinline__ RecF rec_fields_1(const Seg32 &s) {
-    return RecF{to_vgpr(s.lx), to_vgpr(s.ly), to_vgpr(s.lz), s.near2b, s.il2, s.g, s.core,
-                s.near2};
+    return RecF{s.lx, s.ly, s.lz, s.near2b, s.il2, s.g, s.core, s.near2};
 }
 
 // Near pairs go to a per-wave list in LDS instead of a branch in the pair
