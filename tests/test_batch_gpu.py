@@ -79,6 +79,34 @@ def test_autotune_keeps_results(need_gpu):
     assert float((u - ref).abs().max()) <= 1e-7 * float(ref.abs().max())
 
 
+@pytest.mark.parametrize("core", ["vatistas3", "vatistas1_co", "cutoff_mex"])
+def test_batch_other_cores(need_gpu, core):
+    """The persistent kernel's other instances (general Vatistas order,
+    Vatistas n = 1 with the far cut-off, cut-off core with MEX semantics)
+    against each wake's own FP64 evaluation."""
+    import torch
+    from winds.batch import Batch
+    from winds.device import Evaluator
+    if core == "vatistas3":
+        kw = dict(core=winds.CORE_VATISTAS, vat_n=3, semantics=winds.SEM_MATLAB)
+    elif core == "vatistas1_co":
+        kw = dict(core=winds.CORE_VATISTAS, vat_n=1, semantics=winds.SEM_MATLAB, co=40.0)
+    else:
+        kw = dict(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, d=1e-4)
+    wakes = small_batch_wakes()[:4]
+    b = Batch(wakes, winds.opts(precision=winds.PREC_FP32C, **kw),
+              tune={"block": 256, "chunk": 2048})
+    u = b.eval(use_graph=True).clone()
+    ev = Evaluator(0)
+    for w, uw in zip(wakes, b.split(u)):
+        tg = torch.from_numpy(layout.wake_targets(w)).cuda().contiguous()
+        sg = torch.from_numpy(layout.wake_segments(w)).cuda().contiguous()
+        ref = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP64, **kw)).cpu().numpy()
+        s = stats(uw.cpu().numpy(), ref)
+        flush_print(f"   [{core}] wake nt={w.n_targets}: {s}")
+        assert s["max_rel"] <= 1e-6 and s["normwise"] <= 1e-7
+
+
 def test_empty_and_tiny_wakes(need_gpu):
     import torch
     from winds.batch import Batch
