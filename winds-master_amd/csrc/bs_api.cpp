@@ -169,12 +169,11 @@ int to_cfg(const winds_bs_opts *o, winds::EvalCfg &c) {
 int pick_split(int64_t nt, int64_t nseg, const winds::EvalCfg &c) {
     const int64_t waves = ((nt + 63) / 64);
     const int64_t tiles = winds::seg32_padded(nseg) / winds::kSegTile;
-    // SIMDs x resident waves: the kernel runs 8 waves per SIMD, 4 for the
-    // general Vatistas order (sized for 4 waves where 7 ran, the surrogate's
-    // near-wake sum, 6,696 targets x 26,259 segments, took 324 us at 39
-    // splits against 243 us at 68)
-    const bool general = c.core != 0 && c.vat_n != 1 && c.vat_n != 2;
-    const int64_t slots = (int64_t)cu_count() * 4 * (general ? 4 : 8);
+    // SIMDs x resident waves: the kernel runs 8 waves per SIMD (sized for 4
+    // waves where 7 ran, the surrogate's near-wake sum, 6,696 targets x
+    // 26,259 segments, took 324 us at 39 splits against 243 us at 68)
+    (void)c;
+    const int64_t slots = (int64_t)cu_count() * 4 * 8;
     int best = 1;
     double best_cost = 1e300;
     for (int s = 1; s <= 1024; ++s) {

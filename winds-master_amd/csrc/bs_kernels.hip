@@ -128,9 +128,10 @@ __global__ void prep_seg32(const Seg64 *__restrict__ seg, int64_t nseg,
 
 // One launch over a single segment set: grid.x covers targets, grid.y
 // splits the segment range; split partials go to out[split][3][stride].
-// 8 waves per SIMD (at most 64 VGPRs; Vatistas n = 1, 2 spill 24 bytes
-// outside the pair loop): 5-6% faster than 7 waves at 72 VGPRs; the general
-// Vatistas order, 4 waves before its near pairs dropped pow, 45% faster.
+// 8 waves per SIMD (at most 64 VGPRs; 12-48 bytes per lane spill, stored
+// and reloaded once per 16-record tile, outside the pair loop): 5-6% faster
+// than 7 waves at 72 VGPRs; the general Vatistas order, 4 waves before its
+// near pairs dropped pow, 45% faster.
 template <int CORE, int SEM, bool CO>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void bs_fp32c(
     const double *__restrict__ tx, const double *__restrict__ ty,
