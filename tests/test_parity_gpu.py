@@ -126,6 +126,28 @@ def test_fp32c_config1_whole_wake(c1, core, chained):
         assert ss["max_rel"] <= 1e-6 and ss["rms_rel"] <= 1e-7 and ss["normwise"] <= 1e-7
 
 
+@pytest.mark.parametrize("co", [25.0, 60.0])
+def test_fp32c_far_cutoff_whole_wake(c1, co):
+    """The far cut-off co (BiotSavart.m:111) drops a pair's whole term when
+    r1 > co and r2 > co.  With co inside the wake many pairs sit at the step;
+    pairs within FP32 rounding of it take the FP64 branch, so the FP32 sum
+    keeps or drops the same terms as the FP64 reference."""
+    import torch
+    from winds.device import Evaluator
+    recs = layout.wake_segments(c1, chained=True)
+    tg = torch.from_numpy(layout.wake_targets(c1)).cuda().contiguous()
+    sg = torch.from_numpy(recs).cuda().contiguous()
+    F1 = np.asfortranarray(recs[:, 0:3].reshape(-1, 3, 1, 1))
+    F2 = np.asfortranarray(recs[:, 3:6].reshape(-1, 3, 1, 1))
+    ref = O.bs_matlab(F1, F2, c1.domain, recs[:, 6], recs[:, 7], "visc1", co=co)
+    o = winds.opts(core=winds.CORE_VATISTAS, vat_n=1, semantics=winds.SEM_MATLAB, co=co,
+                   precision=winds.PREC_FP32C)
+    u = layout.uind_to_cm(Evaluator(0).eval(tg, sg, o).cpu().numpy(), c1.domain.shape)
+    s = stats(u, ref)
+    flush_print(f"[fp32c far cut-off co={co}] {s}")
+    assert s["max_rel"] <= 1e-6 and s["normwise"] <= 1e-7
+
+
 # --------------------------------------------------------------------------
 # edge cases of the boundary
 # --------------------------------------------------------------------------

@@ -317,6 +317,8 @@ int enqueue(const winds_batch *b, const double *t3, const double *segs, double *
     WB_HIP(hipGetLastError());
     winds::Fp32Params P;
     P.co2 = (c.semantics == 1 && c.co < 1.0e18) ? (float)(c.co * c.co) : __builtin_inff();
+    // r^2 in the FP32 path is within a few 1e-7 of its FP64 value
+    P.co2_tol = P.co2 * 1.0e-5f;
     P.neg_inv_n = n > 0 ? -1.0f / (float)n : 0.0f;
     P.vat_n = n;
     P.d = c.d;

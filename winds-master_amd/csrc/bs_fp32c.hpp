@@ -108,6 +108,7 @@ __device__ __forceinline__ void two_sum_acc(float &hi, float &lo, float v) {
 // Parameters of the FP32 kernels that are not per segment.
 struct Fp32Params {
     float co2;        // co^2 (CO kernels)
+    float co2_tol;    // |r^2 - co^2| at or below which a pair goes to FP64
     float neg_inv_n;  // -1/n (general Vatistas order)
     int vat_n;
     double d;         // cut-off (near pairs)
@@ -297,7 +298,16 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     // the near mask combined from compare masks in SGPRs (a ballot of a
     // combined per-lane condition would go through a VGPR and a compare)
     const uint64_t m3 = __builtin_amdgcn_ballot_w64(n3);
-    const uint64_t m = __builtin_amdgcn_ballot_w64(n1) & (mc | m3);
+    uint64_t m = __builtin_amdgcn_ballot_w64(n1) & (mc | m3);
+    if constexpr (CO) {
+        // The far cut-off (zero when r1 > co and r2 > co) is a step: a pair
+        // with r1 or r2 within FP32 rounding of co could fall on the other
+        // side of it than in FP64 and drop or keep a whole term, so such
+        // pairs take the FP64 branch, which decides as the reference does.
+        const bool ca = !(fabsf(a.s - P.co2) > P.co2_tol);
+        const bool cb = !(fabsf(b.s - P.co2) > P.co2_tol);
+        m |= __builtin_amdgcn_ballot_w64(ca) | __builtin_amdgcn_ballot_w64(cb);
+    }
     const bool near = __builtin_amdgcn_inverse_ballot_w64(m);
     mc = m3;
     // r1 r2 + r1.r2: from the carried lengths (one product per point, where
