@@ -71,6 +71,11 @@ typedef struct winds_bs_opts {
                            /*   > 1 = split sums (for few targets)           */
     int32_t seg_col;       /* SEM_MATLAB+FP64: segments per station column   */
     int32_t seg_blk;       /*   and per blade, for the nested MATLAB sum     */
+    double guard_kappa;    /* FP32C cancellation guard: a target whose FP32  */
+                           /*   terms' 16-segment partial sums have a root   */
+                           /*   sum of squares above guard_kappa * |u| is    */
+                           /*   evaluated again in FP64 over all segments;   */
+                           /*   0 selects 1.0, negative turns the guard off  */
 } winds_bs_opts;
 
 /* Fill o with the defaults: cut-off, d = 1e-4, MEX semantics, FP64. */
@@ -88,7 +93,10 @@ const char *winds_version(void);
  *  F1, F2  [fdims] = [ns,3,nt,nb] column-major doubles, segment endpoints
  *  P       [pdims] = [ns',3,nt',nb'] column-major doubles, targets
  *          (callers pad 2-D/3-D P to 4-D, BiotSavartMex.cu:745-764)
- *  gamma   [ns,1,nt,nb] circulation;  rc [ns,1,nt,nb] squared core radius
+ *  gamma   [ns,1,nt,nb] circulation (ns*nt*nb values are read)
+ *  rc      [ns,1,nt,nb] squared core radius (ns*nt*nb values are read), or
+ *          NULL: then every r_c is 0 (allowed for the cut-off cores only;
+ *          the FP32C near radius is then max(0.51|L|, 0))
  *  d       cut-off (cmod "none"/"cutm") or Vatistas order n (cmod "visc")
  *  cmod    "none": cut-off core, MEX semantics (what BiotSavartMex computes)
  *          "cutm": cut-off core, BiotSavart.m semantics (zeroing, co)
@@ -127,6 +135,15 @@ int winds_bs_eval_dev(const double *targets, int64_t n_targets,
                       size_t workspace_bytes, void *stream);
 
 /*
+ * Number of targets the FP32C cancellation guard evaluated again in FP64 in
+ * the last evaluation that used this workspace with these sizes and options
+ * (0 for FP64 or with the guard off).  Synchronises stream.  No counterpart
+ * in the reference (it has no FP32 path).
+ */
+int winds_bs_guard_count(const void *workspace, int64_t n_targets, int64_t n_segments,
+                         const winds_bs_opts *o, int64_t *count, void *stream);
+
+/*
  * Column-major [ns,3,nt,nb] endpoint arrays (device) to segment records
  * (device), in BiotSavartMex's packed order k = m + n*ns + p*ns*nt
  * (crunch_f/crunch_rg, BiotSavartMex.cu:316-357).
@@ -163,8 +180,15 @@ int winds_device_count(void);
  * SoA over all targets.  The plan holds the work list (wake, target tile,
  * segment chunk) in longest-first order; a persistent kernel pulls items
  * from an atomic counter; per-item partial sums are reduced per target in a
- * fixed order, so results do not depend on scheduling.  The precision must
- * be WINDS_PREC_FP32C.
+ * fixed order, so results do not depend on scheduling.
+ *
+ * WINDS_PREC_FP32C: FP32 compensated sums with FP64 near pairs and the
+ * cancellation guard (guard_kappa).  WINDS_PREC_FP64: one item per target
+ * tile covering the wake's whole segment range, each target summed in
+ * segment order with the reference arithmetic of the semantics selected
+ * (biot_p2p, BiotSavartMex.cu:63-93, 268-282; BiotSavart.m:86-115), so each
+ * wake's result is bitwise equal to winds_bs_eval_dev in FP64 on that wake
+ * alone (seg_col = seg_blk = 0).  FP64 needs no workspace beyond 256 bytes.
  */
 typedef struct winds_batch winds_batch;
 
@@ -193,6 +217,11 @@ int winds_batch_autotune(winds_batch *b, const double *targets, const double *se
                          double *best_ms);
 /* Totals: targets, segments, interactions (sum over wakes of nt*ns), items. */
 int winds_batch_info(const winds_batch *b, int64_t out[4]);
+/* Targets the FP32C guard evaluated again in FP64 in the last evaluation
+ * with this workspace (0 for FP64 or with the guard off).  Synchronises
+ * stream. */
+int winds_batch_guard_count(const winds_batch *b, const void *workspace, int64_t *count,
+                            void *stream);
 
 #ifdef __cplusplus
 }

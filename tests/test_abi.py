@@ -50,6 +50,7 @@ def test_version_and_defaults():
     L.winds_bs_opts_default(C.byref(o))
     assert o.core == 0 and o.semantics == 0 and o.precision == 0
     assert o.d == 1e-4 and math.isinf(o.co) and o.near_factor == 40.0 and o.near_len_frac == 0.51
+    assert o.guard_kappa == 1.0
 
 
 def _call(type="fuln", cmod="none", d=1e-4, fd=(2, 3, 2, 1), pd=(2, 3, 2, 1), gpuhw=(256, 1, 0)):
@@ -65,20 +66,29 @@ def _call(type="fuln", cmod="none", d=1e-4, fd=(2, 3, 2, 1), pd=(2, 3, 2, 1), gp
 
 
 def test_argument_errors_before_device():
-    assert _call(type="xxxx") == -3
-    assert _call(cmod="bogus") == -3
-    assert _call(cmod="visc", d=2.5) == -3          # Vatistas order must be an integer
-    assert _call(fd=(2, 2, 2, 1)) == -2              # F must be [ns,3,nt,nb]
-    assert _call(pd=(2, 4, 2, 1)) == -2
-    assert _call(gpuhw=(100, 1, 0)) == -1            # threads not a multiple of 64
-    assert _call(gpuhw=(256, 2, 0)) == -1            # one GPU per call
-    assert b"threads" in _lib.lib().winds_last_error() or True
+    err = lambda: _lib.lib().winds_last_error().decode()
+    assert _call(type="xxxx") == -3 and "unknown type 'xxxx'" in err()
+    assert _call(cmod="bogus") == -3 and "unknown cmod 'bogus'" in err()
+    assert _call(cmod="visc", d=2.5) == -3 and "integer Vatistas order" in err()
+    assert _call(fd=(2, 2, 2, 1)) == -2 and "F1/F2 must be [ns,3,nt,nb]" in err()
+    assert _call(pd=(2, 4, 2, 1)) == -2 and "P must be [ns,3,nt,nb]" in err()
+    assert _call(gpuhw=(100, 1, 0)) == -1 and "threads must be 0 or a multiple of 64" in err()
+    assert _call(gpuhw=(256, 2, 0)) == -1 and "one GPU per call" in err()
+    # a failing call leaves its own message, not the previous one
+    assert _call(type="yyyy") == -3 and "unknown type 'yyyy'" in err()
 
 
 def test_device_opts_validation():
     L = _lib.lib()
     o = _lib.opts(core=winds.CORE_VATISTAS, semantics=winds.SEM_MEX)
     assert L.winds_bs_workspace_bytes(10, 10, C.byref(o)) == 0
+    o = _lib.opts(precision=winds.PREC_FP32C, guard_kappa=float("nan"))
+    assert L.winds_bs_workspace_bytes(10, 10, C.byref(o)) == 0
+    # the guard's buffers are part of the FP32C workspace; off, they are not
+    on = L.winds_bs_workspace_bytes(1000, 5000, C.byref(_lib.opts(precision=winds.PREC_FP32C)))
+    off = L.winds_bs_workspace_bytes(1000, 5000, C.byref(_lib.opts(precision=winds.PREC_FP32C,
+                                                                   guard_kappa=-1.0)))
+    assert on > off > 0
     o = _lib.opts(precision=winds.PREC_FP32C)
     assert L.winds_bs_workspace_bytes(1000, 5000, C.byref(o)) >= 5024 * 80
 

@@ -109,6 +109,22 @@ def test_argument_count_and_strings(mex):
     A.free()
 
 
+def test_gamma_and_rc_sizes_are_checked(mex):
+    """gamma and rc must hold one value per segment (the engine reads that
+    many): a short gamma or a scalar rc is an error, not a heap over-read."""
+    A = Args(mex)
+    w = gen.rotor_wake(nb=2, nst=5, ages=6)
+    bad = small_call(A, w)
+    bad[3] = A.num(np.ones(3))
+    rc, msg, _ = A.call(bad)
+    assert rc == 1 and msg == "gamma must hold one value per segment."
+    bad = small_call(A, w)
+    bad[4] = A.num(0.0)
+    rc, msg, _ = A.call(bad)
+    assert rc == 1 and msg == "rc must be empty or hold one value per segment."
+    A.free()
+
+
 def test_engine_errors_reach_matlab(mex):
     A = Args(mex)
     w = gen.rotor_wake(nb=2, nst=5, ages=6)
@@ -141,4 +157,10 @@ def test_gateway_matches_the_oracle(need_gpu, mex):
     u2 = A.array(outs[0])
     assert u2.shape == (P2.shape[0], 3, 1, 1)
     assert np.array_equal(u2[:, :, 0, 0], ref[:, :, 0, 0])
+    # an empty rc is accepted for the cut-off core (r_c = 0 everywhere)
+    ins = small_call(A, w)
+    ins[4] = A.num(np.zeros((0, 0)))
+    rc_, msg, outs = A.call(ins)
+    assert rc_ == 0, msg
+    assert np.array_equal(A.array(outs[0]), ref)
     A.free()

@@ -426,3 +426,124 @@ def test_config2_fp32c_within_1e6_of_fp64(c2_device, core):
     else:
         ref = O.bs_matlab(F1, F2, P, recs[:, 6], recs[:, 7], "visc2")
     assert np.array_equal(u64[:, idx].T, ref[:, :, 0, 0])
+
+
+# --------------------------------------------------------------------------
+# FP32 cancellation guard, general Vatistas order at large n, co < 0
+# --------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def wake60(need_gpu):
+    """Wake 60 of the 64-wake uneven batch at full size (3 blades x 20
+    stations x 1,449 ages: 86,940 targets x 169,473 segments).  One target
+    of it (|u| = 8e-5 of the wake's largest) has FP32 terms whose root sum
+    of squares is ~400 |u|: no FP32 arithmetic reaches 1e-6 there."""
+    import torch
+    w = gen.rotor_wake(**gen.config3_params()[60])
+    tg = torch.from_numpy(layout.wake_targets(w)).cuda().contiguous()
+    sg = torch.from_numpy(layout.wake_segments(w, chained=True)).cuda().contiguous()
+    return w, tg, sg
+
+
+@pytest.mark.parametrize("core", ["vatistas2", "cutoff"])
+def test_guard_holds_1e6_per_target_on_cancelling_targets(wake60, core):
+    from winds.device import Evaluator
+    w, tg, sg = wake60
+    ev = Evaluator(0)
+    if core == "cutoff":
+        kw = dict(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, d=1e-4)
+    else:
+        kw = dict(core=winds.CORE_VATISTAS, vat_n=2, semantics=winds.SEM_MATLAB)
+    flush_print(f"[guard {core}] fp64")
+    u64 = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP64, **kw)).cpu().numpy()
+    flush_print(f"[guard {core}] fp32c, guard on")
+    u32 = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP32C, **kw)).cpu().numpy()
+    n_fix = ev.guard_count()
+    s = stats(u32, u64)
+    flush_print(f"   guard on: {s}, {n_fix} targets re-evaluated in FP64")
+    assert s["max_rel"] <= 1e-6 and s["rms_rel"] <= 1e-7 and s["normwise"] <= 1e-7
+    assert 0 < n_fix <= 2e-3 * w.n_targets
+    # the same call again: the same bits (the list order may differ, each
+    # listed target's FP64 sum does not)
+    again = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP32C, **kw)).cpu().numpy()
+    assert np.array_equal(u32, again)
+    flush_print(f"[guard {core}] fp32c, guard off")
+    off = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP32C, guard_kappa=-1.0,
+                                     **kw)).cpu().numpy()
+    assert ev.guard_count() == 0
+    so = stats(off, u64)
+    flush_print(f"   guard off: {so}")
+    # the guard changes only the listed targets, and only towards FP64
+    changed = np.any(off != u32, axis=0)
+    assert changed.sum() <= n_fix
+    if core == "vatistas2":
+        assert so["max_rel"] > 1e-6     # what the guard is for
+
+
+def test_guard_uses_every_fix_path(need_gpu):
+    """kappa = 1e-6 lists every target of a small wake: more than the split
+    re-evaluation's capacity, so the later list entries take the one-
+    workgroup path.  Both paths add the same chunks in the same order: the
+    result is the FP64 chunked sum for every listed target whichever path
+    it took, and repeated calls agree bit for bit."""
+    import torch
+    from winds.device import Evaluator
+    w = gen.rotor_wake(nb=3, nst=18, ages=120)
+    tg = torch.from_numpy(layout.wake_targets(w)).cuda().contiguous()
+    sg = torch.from_numpy(layout.wake_segments(w, chained=True)).cuda().contiguous()
+    ev = Evaluator(0)
+    kw = dict(core=winds.CORE_VATISTAS, vat_n=2, semantics=winds.SEM_MATLAB)
+    u64 = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP64, **kw)).cpu().numpy()
+    o = winds.opts(precision=winds.PREC_FP32C, guard_kappa=1e-6, **kw)
+    a = ev.eval(tg, sg, o).cpu().numpy()
+    n_fix = ev.guard_count()
+    flush_print(f"[guard all paths] {n_fix} of {w.n_targets} listed")
+    assert n_fix == w.n_targets > 4096
+    b = ev.eval(tg, sg, o).cpu().numpy()
+    assert np.array_equal(a, b)
+    s = stats(a, u64)
+    flush_print(f"   {s}")
+    assert s["max_rel"] <= 1e-11
+
+
+@pytest.mark.parametrize("n", [8, 10, 64])
+def test_general_vatistas_order_large_n(c1, n):
+    """Vatistas order n in the general (run-time n) arithmetic.  h^2n leaves
+    the FP32 range at n = 10 beyond h ~ 84 m and the FP64 range at n = 64
+    beyond h ~ 259 m (there the reference's term is h^2 * inf^(-1/n) = 0).
+    Targets are the wake's nodes plus nodes moved 150-900 m off the rotor
+    plane, against the FP64 engine (pow, as the reference) and the oracle."""
+    import torch
+    from winds.device import Evaluator
+    recs = layout.wake_segments(c1, chained=True)
+    T = layout.wake_targets(c1)
+    rng = np.random.default_rng(n)
+    far = T[:, ::7] + rng.uniform(150.0, 900.0, size=(1, T[:, ::7].shape[1])) * \
+        np.array([[0.0], [0.6], [0.8]])
+    T = np.ascontiguousarray(np.concatenate([T, far], axis=1))
+    tg = torch.from_numpy(T).cuda()
+    sg = torch.from_numpy(recs).cuda().contiguous()
+    ev = Evaluator(0)
+    kw = dict(core=winds.CORE_VATISTAS, vat_n=n, semantics=winds.SEM_MATLAB)
+    u64 = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP64, **kw)).cpu().numpy()
+    u32 = ev.eval(tg, sg, winds.opts(precision=winds.PREC_FP32C, **kw)).cpu().numpy()
+    assert np.isfinite(u32).all()
+    s = stats(u32, u64)
+    flush_print(f"[vatistas n={n}] {s}, guard {ev.guard_count()}")
+    assert s["max_rel"] <= 1e-6 and s["normwise"] <= 1e-7
+    idx = np.random.default_rng(3).choice(T.shape[1], size=32, replace=False)
+    P = np.asfortranarray(T[:, idx].T.reshape(-1, 3, 1, 1))
+    F1 = np.asfortranarray(recs[:, 0:3].reshape(-1, 3, 1, 1))
+    F2 = np.asfortranarray(recs[:, 3:6].reshape(-1, 3, 1, 1))
+    ref = O.bs_matlab(F1, F2, P, recs[:, 6], recs[:, 7], f"visc{n}")
+    assert stats(u64[:, idx], ref[:, :, 0, 0].T)["max_rel"] <= 1e-12
+
+
+@pytest.mark.parametrize("cmod,d", [("cutm", 1e-4), ("visc", 2)])
+def test_negative_co_zeroes_every_term(c1, cmod, d):
+    """r1 > co and r2 > co holds for every pair when co < 0, so BiotSavart.m
+    zeroes every term (BiotSavart.m:111); the FP32 path does the same."""
+    F1, F2, g, rc = c1.trail_segments()
+    for typ in ("fuln", "fulc"):
+        u, _ = winds.BiotSavartMex(F1, F2, c1.domain, g, rc, d, cmod, -2.0, typ)
+        assert np.all(u == 0), (cmod, typ)

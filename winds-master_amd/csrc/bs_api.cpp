@@ -157,6 +157,8 @@ int to_cfg(const winds_bs_opts *o, winds::EvalCfg &c) {
     c.split = o->split;
     c.seg_col = o->seg_col;
     c.seg_blk = o->seg_blk;
+    if (std::isnan(o->guard_kappa)) return fail(WINDS_E_ARG, "guard_kappa is NaN");
+    c.guard = o->guard_kappa == 0.0 ? winds::kGuardKappa : o->guard_kappa;
     return WINDS_OK;
 }
 
@@ -202,9 +204,19 @@ size_t ws_bytes(int64_t nt, int64_t nseg, const winds::EvalCfg &c, int *split_ou
     }
     const int split = c.split > 0 ? c.split : pick_split(nt, nseg, c);
     if (split_out) *split_out = split;
+    const bool guard = c.guard >= 0.0;
     const size_t s32 = align256((size_t)winds::seg32_padded(nseg) * sizeof(winds::Seg32));
-    const size_t part = split > 1 ? align256((size_t)split * 3 * (size_t)nt * sizeof(double)) : 0;
-    return s32 + part + 256;
+    const size_t part = split > 1 || guard ? align256((size_t)split * 3 * (size_t)nt * sizeof(double)) : 0;
+    const size_t gb = guard ? winds::guard_bytes((int64_t)split * nt, nt) : 0;
+    return s32 + part + gb + 256;
+}
+
+// The guard's buffers inside an FP32C workspace laid out by ws_bytes.
+winds::GuardBufs guard_of(void *ws, int64_t nt, int64_t nseg, int split) {
+    char *base = (char *)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+    char *g = base + align256((size_t)winds::seg32_padded(nseg) * sizeof(winds::Seg32)) +
+              align256((size_t)split * 3 * (size_t)nt * sizeof(double));
+    return winds::guard_carve(g, (int64_t)split * nt, nt);
 }
 
 int eval_dev_impl(const double *t3, int64_t nt, const winds::Seg64 *seg, int64_t nseg,
@@ -229,7 +241,9 @@ int eval_dev_impl(const double *t3, int64_t nt, const winds::Seg64 *seg, int64_t
     char *base = (char *)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
     winds::Seg32 *s32 = (winds::Seg32 *)base;
     double *part = (double *)(base + align256((size_t)winds::seg32_padded(nseg) * sizeof(winds::Seg32)));
-    WINDS_HIP(winds::launch_fp32c(t3, nt, seg, nseg, c, u3, s32, part, st));
+    winds::GuardBufs g;
+    if (c.guard >= 0.0) g = guard_of(ws, nt, nseg, split);
+    WINDS_HIP(winds::launch_fp32c(t3, nt, seg, nseg, c, u3, s32, part, g, st));
     return WINDS_OK;
 }
 
@@ -250,6 +264,7 @@ void winds_bs_opts_default(winds_bs_opts *o) {
     o->co = __builtin_inf();
     o->near_factor = kNearFactor;
     o->near_len_frac = kNearLenFrac;
+    o->guard_kappa = winds::kGuardKappa;
 }
 
 const char *winds_last_error(void) { return g_err.c_str(); }
@@ -282,6 +297,28 @@ int winds_bs_eval_dev(const double *targets, int64_t n_targets, const double *se
         return fail(WINDS_E_DIMS, "more than 2^31 targets or segments");
     return eval_dev_impl(targets, n_targets, (const winds::Seg64 *)segments, n_segments, c, uind,
                          workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int winds_bs_guard_count(const void *workspace, int64_t n_targets, int64_t n_segments,
+                         const winds_bs_opts *o, int64_t *count, void *stream) {
+    g_err.clear();
+    if (!count) return fail(WINDS_E_ARG, "null count");
+    *count = 0;
+    winds::EvalCfg c;
+    int rc = to_cfg(o, c);
+    if (rc) return rc;
+    if (c.precision != WINDS_PREC_FP32C || c.guard < 0.0 || n_targets <= 0 || n_segments <= 0)
+        return WINDS_OK;
+    if (!workspace) return fail(WINDS_E_ARG, "null workspace");
+    int split = 1;
+    (void)ws_bytes(n_targets, n_segments, c, &split);
+    const winds::GuardBufs g = guard_of(const_cast<void *>(workspace), n_targets, n_segments, split);
+    int v = 0;
+    hipStream_t st = (hipStream_t)stream;
+    WINDS_HIP(hipMemcpyAsync(&v, g.count, sizeof(int), hipMemcpyDeviceToHost, st));
+    WINDS_HIP(hipStreamSynchronize(st));
+    *count = v;
+    return WINDS_OK;
 }
 
 int winds_pack_segments_dev(const double *F1, const double *F2, const int64_t fdims[4],

@@ -9,6 +9,14 @@
 // target in chunk order, so the result is independent of scheduling.  The
 // whole evaluation (record preparation, counter reset, persistent kernel,
 // reduce) can be recorded into a hipGraph once and replayed.
+//
+// FP32C: partial sums per (target tile, segment chunk) item, reduced per
+// target in chunk order, then the cancellation guard (targets whose FP32
+// terms cancel are summed again in FP64).  FP64: one item per target tile
+// with the wake's whole segment range, each lane summing its target
+// sequentially in segment order -- the order of BiotSavart_nogpu
+// (BiotSavartMex.cu:268-282) and of the single-wake FP64 kernels -- so every
+// wake's result is bitwise equal to its own FP64 evaluation.
 
 #include <hip/hip_runtime.h>
 
@@ -69,8 +77,11 @@ __global__ void prep_seg32_batch(const Seg64 *__restrict__ seg, const BatchWake 
     out[k] = make_seg32(src, prev, next, core_kind, n, d, near_factor, near_len_frac);
 }
 
-__global__ void reset_counter(int *c) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
+__global__ void reset_counter(int *c, int *guard_count) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *c = 0;
+        if (guard_count) *guard_count = 0;
+    }
 }
 
 // 8 waves per SIMD (at most 64 VGPRs), with the second record of each
@@ -83,7 +94,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void b
     const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
     const Seg32 *__restrict__ s32, const Seg64 *__restrict__ s64,
     const WorkItem *__restrict__ items, int n_items, int *__restrict__ counter, Fp32Params P,
-    double *__restrict__ part) {
+    double *__restrict__ part, float *__restrict__ est) {
     __shared__ int s_item;
     extern __shared__ NearLds near_lds[];  // one per wave
     const int B = blockDim.x;
@@ -100,20 +111,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void b
         const bool active = lane < w.tn;
         const int64_t t = w.t0 + (active ? lane : 0);
         double sx, sy, sz;
+        float sa;
         fp32c_range<CORE, SEM, CO>(tx[t], ty[t], tz[t], s32, s64, w.shift, w.k0, w.k1, P,
-                                   near_lds + (lane >> 6), sx, sy, sz);
+                                   near_lds + (lane >> 6), sx, sy, sz, sa);
         if (active) {
             double *o = part + (int64_t)w.slot * 3 * B;
             o[lane] = sx;
             o[B + lane] = sy;
             o[2 * B + lane] = sz;
+            if (est) est[(int64_t)w.slot * B + lane] = sa;
         }
     }
 }
 
-__global__ void bs_batch_reduce(const double *__restrict__ part, const BatchWake *__restrict__ wk,
-                                int n_wakes, int64_t nt_total, int B, double *__restrict__ ux,
-                                double *__restrict__ uy, double *__restrict__ uz) {
+// FP64 items: every lane sums its target over [k0, k1) of the FP64 records
+// in segment order (fp64_range) and stores the result itself.  MODE 0
+// biot_p2p, 1 BiotSavart.m cut-off, 2 BiotSavart.m Vatistas of order NF (1,
+// 2) or n (NF = 0).  1-2 waves per SIMD, as the single-set FP64 kernel.
+template <int MODE, int NF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void bs_batch_fp64(
+    const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
+    const Seg64 *__restrict__ s64, const WorkItem *__restrict__ items, int n_items,
+    int *__restrict__ counter, double d, int n, double co, double *__restrict__ ux,
+    double *__restrict__ uy, double *__restrict__ uz) {
+    __shared__ int s_item;
+    const int lane = threadIdx.x;
+    for (;;) {
+        if (lane == 0) s_item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (it >= n_items) break;
+        const WorkItem w = items[it];
+        if (lane < w.tn) {
+            const int64_t t = w.t0 + lane;
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            fp64_range<MODE, NF>(s64, w.k0, w.k1, tx[t], ty[t], tz[t], d, n, co, sx, sy, sz);
+            ux[t] = sx;
+            uy[t] = sy;
+            uz[t] = sz;
+        }
+    }
+}
+
+// Slots of each target added in chunk order; with est, the guard's test
+// (as bs_reduce_guard) appends the targets whose FP32 terms cancel to list.
+__global__ void bs_batch_reduce(const double *__restrict__ part, const float *__restrict__ est,
+                                const BatchWake *__restrict__ wk, int n_wakes, int64_t nt_total,
+                                int B, double kappa, double *__restrict__ ux,
+                                double *__restrict__ uy, double *__restrict__ uz,
+                                uint32_t *__restrict__ list, int *__restrict__ count) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nt_total) return;
     const int w = find_wake_by(wk, n_wakes, g, false);
@@ -121,16 +168,20 @@ __global__ void bs_batch_reduce(const double *__restrict__ part, const BatchWake
     const int64_t tile = t / B;
     const int lane = (int)(t % B);
     const int nc = wk[w].nchunk;
-    const double *p = part + (wk[w].slot0 + tile * nc) * 3 * B;
-    double sx = 0.0, sy = 0.0, sz = 0.0;
+    const int64_t s0 = wk[w].slot0 + tile * nc;
+    const double *p = part + s0 * 3 * B;
+    double sx = 0.0, sy = 0.0, sz = 0.0, a2 = 0.0;
     for (int c = 0; c < nc; ++c, p += 3 * B) {
         sx += p[lane];
         sy += p[B + lane];
         sz += p[2 * B + lane];
+        if (est) a2 += (double)est[(s0 + c) * B + lane];
     }
     ux[g] = sx;
     uy[g] = sy;
     uz[g] = sz;
+    if (est && !(sqrt(a2) <= kappa * sqrt(sx * sx + sy * sy + sz * sz)))
+        list[atomicAdd(count, 1)] = (uint32_t)g;
 }
 
 }  // namespace winds
@@ -148,6 +199,7 @@ struct winds_batch {
     int64_t nt_total = 0, ns_total = 0, np_total = 0, inter = 0;
     int64_t n_items = 0, n_slots = 0;
     BatchWake *d_wakes = nullptr;
+    winds::WakeSpan *d_spans = nullptr;
     WorkItem *d_items = nullptr;
     int *d_counter = nullptr;
     int grid = 0;
@@ -176,9 +228,11 @@ void free_plan_device(winds_batch *b) {
     b->exec = nullptr;
     b->graph = nullptr;
     if (b->d_wakes) (void)hipFree(b->d_wakes);
+    if (b->d_spans) (void)hipFree(b->d_spans);
     if (b->d_items) (void)hipFree(b->d_items);
     if (b->d_counter) (void)hipFree(b->d_counter);
     b->d_wakes = nullptr;
+    b->d_spans = nullptr;
     b->d_items = nullptr;
     b->d_counter = nullptr;
     for (auto &k : b->gkey) k = nullptr;
@@ -195,9 +249,26 @@ int occ_blocks(int block) {
     return n;
 }
 
+template <int M, int F>
+int occ_blocks64(int block) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, winds::bs_batch_fp64<M, F>, block, 0) !=
+            hipSuccess ||
+        n < 1)
+        n = std::max(1, 8 / std::max(1, block / 64));
+    return n;
+}
+
 // Resident workgroups per CU of the persistent kernel instance used by cfg.
 int default_wg_per_cu(const winds::EvalCfg &c, int block) {
     const int n = c.core == 0 ? 0 : c.vat_n;
+    if (c.precision == WINDS_PREC_FP64) {
+        if (c.semantics == 0) return occ_blocks64<0, 0>(block);
+        if (n == 0) return occ_blocks64<1, 0>(block);
+        if (n == 1) return occ_blocks64<2, 1>(block);
+        if (n == 2) return occ_blocks64<2, 2>(block);
+        return occ_blocks64<2, 0>(block);
+    }
     const bool co = c.semantics == 1 && c.co < 1.0e18;
     const int kind = c.core == 0 ? 0 : (n == 1 ? 1 : (n == 2 ? 2 : 3));
     if (c.semantics == 0) return occ_blocks<0, 0, false>(block);
@@ -219,7 +290,9 @@ int default_wg_per_cu(const winds::EvalCfg &c, int block) {
 
 int build_plan(winds_batch *b) {
     const int B = b->tune.block;
-    const int64_t chunk = b->tune.chunk;
+    const bool fp64 = b->cfg.precision == WINDS_PREC_FP64;
+    // FP64: the whole segment range per item (the reference's sequential order)
+    const int64_t chunk = fp64 ? INT64_MAX / 4 : b->tune.chunk;
     b->hw.assign(b->n_wakes, BatchWake{});
     int64_t toff = 0, soff = 0, poff = 0, slot = 0, inter = 0;
     for (int w = 0; w < b->n_wakes; ++w) {
@@ -229,7 +302,7 @@ int build_plan(winds_batch *b) {
         x.s_off = soff;
         x.ns = b->ns[w];
         x.p_off = poff;
-        x.np = winds::seg32_padded(x.ns);
+        x.np = fp64 ? x.ns : winds::seg32_padded(x.ns);
         x.ntile = (int32_t)((x.nt + B - 1) / B);
         x.nchunk = x.np > 0 ? (int32_t)((x.np + chunk - 1) / chunk) : 0;
         x.slot0 = slot;
@@ -250,7 +323,7 @@ int build_plan(winds_batch *b) {
     for (int w = 0; w < b->n_wakes; ++w) {
         const BatchWake &x = b->hw[w];
         for (int c = 0; c < x.nchunk; ++c) {
-            const int64_t k0 = x.p_off + (int64_t)c * chunk;
+            const int64_t k0 = x.p_off + (int64_t)c * std::min(chunk, x.np);
             const int64_t k1 = std::min(k0 + chunk, x.p_off + x.np);
             for (int t = 0; t < x.ntile; ++t) {
                 WorkItem it;
@@ -270,10 +343,16 @@ int build_plan(winds_batch *b) {
     });
     b->n_items = (int64_t)items.size();
     free_plan_device(b);
+    std::vector<winds::WakeSpan> spans(b->n_wakes);
+    for (int w = 0; w < b->n_wakes; ++w)
+        spans[w] = winds::WakeSpan{b->hw[w].t_off, b->hw[w].nt, b->hw[w].s_off, b->hw[w].ns};
     WB_HIP(hipMalloc(&b->d_wakes, sizeof(BatchWake) * std::max(1, b->n_wakes)));
+    WB_HIP(hipMalloc(&b->d_spans, sizeof(winds::WakeSpan) * std::max(1, b->n_wakes)));
     WB_HIP(hipMalloc(&b->d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
     WB_HIP(hipMalloc(&b->d_counter, 256));
     WB_HIP(hipMemcpy(b->d_wakes, b->hw.data(), sizeof(BatchWake) * b->n_wakes, hipMemcpyHostToDevice));
+    WB_HIP(hipMemcpy(b->d_spans, spans.data(), sizeof(winds::WakeSpan) * b->n_wakes,
+                     hipMemcpyHostToDevice));
     if (!items.empty())
         WB_HIP(hipMemcpy(b->d_items, items.data(), sizeof(WorkItem) * items.size(),
                          hipMemcpyHostToDevice));
@@ -282,73 +361,119 @@ int build_plan(winds_batch *b) {
     return WINDS_OK;
 }
 
+bool guarded(const winds_batch *b) {
+    return b->cfg.precision == WINDS_PREC_FP32C && b->cfg.guard >= 0.0;
+}
+
 size_t ws_need(const winds_batch *b) {
+    if (b->cfg.precision == WINDS_PREC_FP64) return 256;
+    const size_t n_est = (size_t)b->n_slots * b->tune.block;
     return a256((size_t)b->np_total * sizeof(winds::Seg32)) +
-           a256((size_t)b->n_slots * 3 * b->tune.block * sizeof(double)) + 256;
+           a256((size_t)b->n_slots * 3 * b->tune.block * sizeof(double)) +
+           (guarded(b) ? winds::guard_bytes((int64_t)n_est, b->nt_total) : 0) + 256;
+}
+
+winds::GuardBufs guard_of(const winds_batch *b, void *ws) {
+    char *base = (char *)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
+    char *g = base + a256((size_t)b->np_total * sizeof(winds::Seg32)) +
+              a256((size_t)b->n_slots * 3 * b->tune.block * sizeof(double));
+    return winds::guard_carve(g, (int64_t)b->n_slots * b->tune.block, b->nt_total);
 }
 
 template <int K, int S, bool C>
 void launch_persistent(const winds_batch *b, const double *t3, const winds::Seg32 *s32,
                        const winds::Seg64 *s64, const winds::Fp32Params &P, double *part,
-                       hipStream_t st) {
+                       float *est, hipStream_t st) {
     winds::bs_batch_persistent<K, S, C><<<b->grid, b->tune.block,
                                           (size_t)(b->tune.block / 64) * sizeof(winds::NearLds),
                                           st>>>(
         t3, t3 + b->nt_total, t3 + 2 * b->nt_total, s32, s64, b->d_items, (int)b->n_items,
-        b->d_counter, P, part);
+        b->d_counter, P, part, est);
+}
+
+int enqueue_fp64(const winds_batch *b, const double *t3, const double *segs, double *u3,
+                 hipStream_t st) {
+    const winds::EvalCfg &c = b->cfg;
+    const winds::Seg64 *s64 = (const winds::Seg64 *)segs;
+    const int n = c.core == 0 ? 0 : c.vat_n;
+    const int64_t N = b->nt_total;
+    // wakes without segments keep u = 0: zero everything first (a kernel,
+    // not a memset, so that graph capture records no memset node)
+    WB_HIP(winds::launch_zero(u3, 3 * N, st));
+    winds::reset_counter<<<1, 64, 0, st>>>(b->d_counter, nullptr);
+    WB_HIP(hipGetLastError());
+    if (b->n_items == 0) return WINDS_OK;
+#define WB_FP64(M, F)                                                                          \
+    winds::bs_batch_fp64<M, F><<<b->grid, b->tune.block, 0, st>>>(                               \
+        t3, t3 + N, t3 + 2 * N, s64, b->d_items, (int)b->n_items, b->d_counter, c.d, n, c.co, u3, \
+        u3 + N, u3 + 2 * N)
+    if (c.semantics == 0)
+        WB_FP64(0, 0);
+    else if (n == 0)
+        WB_FP64(1, 0);
+    else if (n == 1)
+        WB_FP64(2, 1);
+    else if (n == 2)
+        WB_FP64(2, 2);
+    else
+        WB_FP64(2, 0);
+#undef WB_FP64
+    WB_HIP(hipGetLastError());
+    return WINDS_OK;
 }
 
 int enqueue(const winds_batch *b, const double *t3, const double *segs, double *u3, void *ws,
             hipStream_t st) {
     const winds::EvalCfg &c = b->cfg;
+    if (b->nt_total == 0) return WINDS_OK;
+    if (c.precision == WINDS_PREC_FP64) return enqueue_fp64(b, t3, segs, u3, st);
     char *base = (char *)(((uintptr_t)ws + 255) & ~(uintptr_t)255);
     winds::Seg32 *s32 = (winds::Seg32 *)base;
     double *part = (double *)(base + a256((size_t)b->np_total * sizeof(winds::Seg32)));
     const winds::Seg64 *s64 = (const winds::Seg64 *)segs;
     const int n = c.core == 0 ? 0 : c.vat_n;
-    if (b->nt_total == 0) return WINDS_OK;
+    const bool guard = guarded(b);
+    winds::GuardBufs g;
+    if (guard) g = guard_of(b, ws);
     if (b->np_total > 0) {
         winds::prep_seg32_batch<<<(unsigned)((b->np_total + 255) / 256), 256, 0, st>>>(
             s64, b->d_wakes, b->n_wakes, b->np_total, c.core == 0 ? 0 : 1, n, c.d, c.near_factor,
             c.near_len_frac, s32);
         WB_HIP(hipGetLastError());
     }
-    winds::reset_counter<<<1, 64, 0, st>>>(b->d_counter);
+    winds::reset_counter<<<1, 64, 0, st>>>(b->d_counter, guard ? g.count : nullptr);
     WB_HIP(hipGetLastError());
-    winds::Fp32Params P;
-    P.co2 = (c.semantics == 1 && c.co < 1.0e18) ? (float)(c.co * c.co) : __builtin_inff();
-    // r^2 in the FP32 path is within a few 1e-7 of its FP64 value
-    P.co2_tol = P.co2 * 1.0e-5f;
-    P.neg_inv_n = n > 0 ? -1.0f / (float)n : 0.0f;
-    P.vat_n = n;
-    P.d = c.d;
-    P.co = c.co;
+    const winds::Fp32Params P = winds::fp32_params(c);
+    float *est = guard ? g.est : nullptr;
     const bool has_co = c.semantics == 1 && c.co < 1.0e18;
     const int kind = c.core == 0 ? 0 : (n == 1 ? 1 : (n == 2 ? 2 : 3));
     if (b->n_items > 0) {
         if (c.semantics == 0) {
-            launch_persistent<0, 0, false>(b, t3, s32, s64, P, part, st);
+            launch_persistent<0, 0, false>(b, t3, s32, s64, P, part, est, st);
         } else if (has_co) {
             switch (kind) {
-                case 0: launch_persistent<0, 1, true>(b, t3, s32, s64, P, part, st); break;
-                case 1: launch_persistent<1, 1, true>(b, t3, s32, s64, P, part, st); break;
-                case 2: launch_persistent<2, 1, true>(b, t3, s32, s64, P, part, st); break;
-                default: launch_persistent<3, 1, true>(b, t3, s32, s64, P, part, st); break;
+                case 0: launch_persistent<0, 1, true>(b, t3, s32, s64, P, part, est, st); break;
+                case 1: launch_persistent<1, 1, true>(b, t3, s32, s64, P, part, est, st); break;
+                case 2: launch_persistent<2, 1, true>(b, t3, s32, s64, P, part, est, st); break;
+                default: launch_persistent<3, 1, true>(b, t3, s32, s64, P, part, est, st); break;
             }
         } else {
             switch (kind) {
-                case 0: launch_persistent<0, 1, false>(b, t3, s32, s64, P, part, st); break;
-                case 1: launch_persistent<1, 1, false>(b, t3, s32, s64, P, part, st); break;
-                case 2: launch_persistent<2, 1, false>(b, t3, s32, s64, P, part, st); break;
-                default: launch_persistent<3, 1, false>(b, t3, s32, s64, P, part, st); break;
+                case 0: launch_persistent<0, 1, false>(b, t3, s32, s64, P, part, est, st); break;
+                case 1: launch_persistent<1, 1, false>(b, t3, s32, s64, P, part, est, st); break;
+                case 2: launch_persistent<2, 1, false>(b, t3, s32, s64, P, part, est, st); break;
+                default: launch_persistent<3, 1, false>(b, t3, s32, s64, P, part, est, st); break;
             }
         }
         WB_HIP(hipGetLastError());
     }
     winds::bs_batch_reduce<<<(unsigned)((b->nt_total + 255) / 256), 256, 0, st>>>(
-        part, b->d_wakes, b->n_wakes, b->nt_total, b->tune.block, u3, u3 + b->nt_total,
-        u3 + 2 * b->nt_total);
+        part, est, b->d_wakes, b->n_wakes, b->nt_total, b->tune.block, c.guard, u3,
+        u3 + b->nt_total, u3 + 2 * b->nt_total, g.list, g.count);
     WB_HIP(hipGetLastError());
+    if (guard)
+        WB_HIP(winds::launch_guard_fix(t3, b->nt_total, s64, b->d_spans, b->n_wakes,
+                                       winds::WakeSpan{0, 0, 0, 0}, c, g, u3, st));
     return WINDS_OK;
 }
 
@@ -375,8 +500,6 @@ int winds_batch_create(int32_t n_wakes, const int64_t *n_targets, const int64_t 
     winds::EvalCfg c;
     int rc = winds::opts_to_cfg(o, c);
     if (rc) return rc;
-    if (c.precision != WINDS_PREC_FP32C)
-        return set_error(WINDS_E_MODE, "batched evaluation runs the FP32 compensated kernel");
     for (int w = 0; w < n_wakes; ++w)
         if (n_targets[w] < 0 || n_segments[w] < 0) return set_error(WINDS_E_DIMS, "negative size");
     winds_batch_tune t{256, 16384, 0, 0};
@@ -416,6 +539,22 @@ void winds_batch_destroy(winds_batch *b) {
 }
 
 size_t winds_batch_workspace_bytes(const winds_batch *b) { return b ? ws_need(b) : 0; }
+
+int winds_batch_guard_count(const winds_batch *b, const void *workspace, int64_t *count,
+                            void *stream) {
+    winds::clear_error();
+    if (!b || !count) return set_error(WINDS_E_ARG, "null argument");
+    *count = 0;
+    if (!guarded(b) || b->nt_total == 0) return WINDS_OK;
+    if (!workspace) return set_error(WINDS_E_ARG, "null workspace");
+    const winds::GuardBufs g = guard_of(b, const_cast<void *>(workspace));
+    int v = 0;
+    hipStream_t st = (hipStream_t)stream;
+    WB_HIP(hipMemcpyAsync(&v, g.count, sizeof(int), hipMemcpyDeviceToHost, st));
+    WB_HIP(hipStreamSynchronize(st));
+    *count = v;
+    return WINDS_OK;
+}
 
 int winds_batch_info(const winds_batch *b, int64_t out[4]) {
     if (!b || !out) return set_error(WINDS_E_ARG, "null argument");
@@ -477,8 +616,11 @@ int winds_batch_autotune(winds_batch *b, const double *targets, const double *se
     WB_HIP(hipEventCreate(&e1));
     int rc = WINDS_OK;
     bool first = true;
+    // FP64 items take whole wakes: the chunk does not apply
+    const int n_chunks = b->cfg.precision == WINDS_PREC_FP64 ? 1 : (int)(sizeof chunks / sizeof *chunks);
     for (int B : blocks) {
-        for (int ch : chunks) {
+        for (int ci = 0; ci < n_chunks; ++ci) {
+            const int ch = chunks[ci];
             b->tune = winds_batch_tune{B, ch, 0, 0};
             rc = build_plan(b);
             if (rc) break;

@@ -73,9 +73,10 @@ __device__ __forceinline__ Seg32 make_seg32(const Seg64 *s_ptr, const Seg64 *pre
     if (core_kind == 0) {
         r.core = (float)(d * l2);
     } else {
-        // FLT_MIN keeps r_c^4 + h^4 > 0 without a per-pair max
-        const float c = (float)ipow(s.rc, n);
-        r.core = c > FLT_MIN ? c : FLT_MIN;
+        // n = 1, 2: r_c^2n; FLT_MIN keeps r_c^4 + h^4 > 0 without a per-pair
+        // max.  n >= 3: r_c^2 itself (the scaled form of the factor).
+        const float c = (float)(n >= 3 ? s.rc : ipow(s.rc, n));
+        r.core = n >= 3 ? c : (c > FLT_MIN ? c : FLT_MIN);
     }
     const double nr2 = near_radius2(s, l2, near_factor, near_len_frac);
     // the cut-off core needs no h^2 = |r1 x r2|^2 / |L|^2 but the near test:
@@ -111,9 +112,14 @@ struct Fp32Params {
     float co2_tol;    // |r^2 - co^2| at or below which a pair goes to FP64
     float neg_inv_n;  // -1/n (general Vatistas order)
     int vat_n;
+    float h2_big;     // general Vatistas order: pairs with h^2 >= h2_big go to FP64
     double d;         // cut-off (near pairs)
     double co;        // far cut-off (near pairs)
 };
+
+struct EvalCfg;
+// Parameters for configuration c (co, order, overflow threshold).
+Fp32Params fp32_params(const EvalCfg &c);
 
 // FP32 view of P - X for a filament point X: the vector, its squared length,
 // the reciprocal of its length and its length.
@@ -308,6 +314,7 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
         const bool cb = !(fabsf(b.s - P.co2) > P.co2_tol);
         m |= __builtin_amdgcn_ballot_w64(ca) | __builtin_amdgcn_ballot_w64(cb);
     }
+    if constexpr (CORE == 3) m |= __builtin_amdgcn_ballot_w64(h2 >= P.h2_big);
     const bool near = __builtin_amdgcn_inverse_ballot_w64(m);
     mc = m3;
     // r1 r2 + r1.r2: from the carried lengths (one product per point, where
@@ -329,17 +336,23 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
         const float w = fmaf(h2, h2, f.core);
         ubar = gn * (h2 * __builtin_amdgcn_rsqf(w * (qa * qa)));
     } else {
-        // h^2/(r_c^2n + h^2n)^(1/n) / (r1r2 + r1.r2)
-        float hn = h2;
-        for (int i = 1; i < P.vat_n; ++i) hn = hn * h2;
-        const float w = fmaxf(f.core + hn, FLT_MIN);
-        // w^(-1/n) from exp2/log2 carries the log's absolute error times
-        // |log2 w| (w ~ 1e-9: ~1e-5 relative); one Newton step on
-        // y^-n = w, y <- y + y (1 - w y^n) / n, brings it to FP32 rounding
+        // h^2/(r_c^2n + h^2n)^(1/n) / (r1r2 + r1.r2), in the scaled form
+        // (h^2/M) (1 + x^n)^(-1/n) with M = max(h^2, r_c^2) and
+        // x = min(h^2, r_c^2)/M <= 1, so that no power leaves the FP32 range
+        // whatever n and h (h^2n itself overflows FP32 at n = 10, h = 100 m)
+        const float M = fmaxf(fmaxf(h2, f.core), FLT_MIN);
+        float iM = __builtin_amdgcn_rcpf(M);
+        iM = fmaf(iM, fmaf(-M, iM, 1.0f), iM);
+        const float x = fminf(h2, f.core) * iM;
+        float xn = x;
+        for (int i = 1; i < P.vat_n; ++i) xn = xn * x;
+        const float w = 1.0f + xn;  // in [1, 2]
+        // w^(-1/n) from exp2/log2, then one Newton step on y^-n = w,
+        // y <- y + y (1 - w y^n) / n, to FP32 rounding
         float root = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(w) * P.neg_inv_n);
         float yn = root;
         for (int i = 1; i < P.vat_n; ++i) yn = yn * root;
-        root = fmaf(root * fmaf(-w, yn, 1.0f), -P.neg_inv_n, root);
+        root = fmaf(root * fmaf(-w, yn, 1.0f), -P.neg_inv_n, root) * iM;
         // 1/qa refined by one Newton step: v_rcp_f32's error was the larger
         // part of this order's (max per-target error on the 1,440-age wake
         // 2.0e-6 -> 1.2e-6, 1% of the time)
@@ -387,8 +400,11 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                                             const Seg32 *__restrict__ s32,
                                             const Seg64 *__restrict__ s64, int64_t shift,
                                             int64_t k0, int64_t k1, const Fp32Params &P,
-                                            NearLds *nlds, double &ox, double &oy, double &oz) {
+                                            NearLds *nlds, double &ox, double &oy, double &oz,
+                                            float &oa) {
     float hx = 0.f, hy = 0.f, hz = 0.f, lx = 0.f, ly = 0.f, lz = 0.f;
+    // sum of |tile partial|^2: how far the FP32 terms cancel at this target
+    float at = 0.f;
     NearList<CORE, SEM> nl;
     nl.init(nlds, s64, shift, k0, px, py, pz);
     // P - (end point of the record before), carried along the filament; the
@@ -429,12 +445,14 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
         two_sum_acc(hx, lx, qx);
         two_sum_acc(hy, ly, qy);
         two_sum_acc(hz, lz, qz);
+        at = fmaf(qz, qz, fmaf(qy, qy, fmaf(qx, qx, at)));
     }
     double ax, ay, az;
     nl.result(P, ax, ay, az);
     ox = ((double)hx + (double)lx) + ax;
     oy = ((double)hy + (double)ly) + ay;
     oz = ((double)hz + (double)lz) + az;
+    oa = at;
 }
 
 }  // namespace winds

@@ -137,7 +137,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void b
     const double *__restrict__ tx, const double *__restrict__ ty,
     const double *__restrict__ tz, int64_t nt, const Seg32 *__restrict__ s32,
     const Seg64 *__restrict__ s64, int64_t nseg_pad, int64_t chunk, Fp32Params P,
-    double *__restrict__ out, int64_t out_stride) {
+    double *__restrict__ out, float *__restrict__ est, int64_t out_stride) {
     extern __shared__ NearLds near_lds[];  // one per wave
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     // whole waves past the end leave; the lanes past the end of a partial
@@ -148,13 +148,154 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void b
     const int64_t k0 = (int64_t)blockIdx.y * chunk;
     const int64_t k1 = k0 + chunk < nseg_pad ? k0 + chunk : nseg_pad;
     double sx, sy, sz;
+    float sa;
     fp32c_range<CORE, SEM, CO>(tx[tc], ty[tc], tz[tc], s32, s64, 0, k0, k1, P,
-                               near_lds + (threadIdx.x >> 6), sx, sy, sz);
+                               near_lds + (threadIdx.x >> 6), sx, sy, sz, sa);
     if (!active) return;
     double *o = out + (int64_t)blockIdx.y * 3 * out_stride;
     o[t] = sx;
     o[out_stride + t] = sy;
     o[2 * out_stride + t] = sz;
+    if (est) est[(int64_t)blockIdx.y * out_stride + t] = sa;
+}
+
+// Split partials of the FP32 kernel summed in split order, as
+// bs_reduce_split, plus the guard's test: a target whose FP32 terms cancel
+// (sqrt(a2) > kappa |u|, or a result that is not finite) is appended to the
+// re-evaluation list.  The list order depends on scheduling; each listed
+// target's FP64 sum does not.
+__global__ void bs_reduce_guard(const double *__restrict__ part, const float *__restrict__ est,
+                                int64_t nt, int split, double kappa, double *__restrict__ ux,
+                                double *__restrict__ uy, double *__restrict__ uz,
+                                uint32_t *__restrict__ list, int *__restrict__ count) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    double sx = 0.0, sy = 0.0, sz = 0.0, a2 = 0.0;
+    for (int s = 0; s < split; ++s) {
+        const double *p = part + (int64_t)s * 3 * nt;
+        sx += p[t];
+        sy += p[nt + t];
+        sz += p[2 * nt + t];
+        a2 += (double)est[(int64_t)s * nt + t];
+    }
+    ux[t] = sx;
+    uy[t] = sy;
+    uz[t] = sz;
+    if (!(sqrt(a2) <= kappa * sqrt(sx * sx + sy * sy + sz * sz)))
+        list[atomicAdd(count, 1)] = (uint32_t)t;
+}
+
+// ---------------------------------------------------------------------------
+// Guard: FP64 re-evaluation of listed targets
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int find_span(const WakeSpan *wk, int n, int64_t t) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (wk[mid].t_off <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Chunk c of kFixSplit of the wake's records at target (px, py, pz): each
+// of the 256 threads sums a contiguous sub-range in segment order, then a
+// fixed tree adds the threads' sums; thread 0 returns the chunk's sum.
+template <int MODE, int NF>
+__device__ double3 fix_chunk(const Seg64 *__restrict__ seg, int64_t ns, int c, double px,
+                             double py, double pz, double d, int n, double co,
+                             double (*red)[256]) {
+    const int64_t k0 = ns * c / kFixSplit, k1 = ns * (c + 1) / kFixSplit;
+    const int64_t len = k1 - k0;
+    const int j = (int)threadIdx.x;
+    const int64_t a = k0 + len * j / 256, b = k0 + len * (j + 1) / 256;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    fp64_range<MODE, NF>(seg, a, b, px, py, pz, d, n, co, sx, sy, sz);
+    __syncthreads();
+    red[0][j] = sx;
+    red[1][j] = sy;
+    red[2][j] = sz;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (j < h) {
+            red[0][j] += red[0][j + h];
+            red[1][j] += red[1][j + h];
+            red[2][j] += red[2][j + h];
+        }
+        __syncthreads();
+    }
+    return make_double3(red[0][0], red[1][0], red[2][0]);
+}
+
+// grid = (kFixSplit, G), 256 threads.  List entry i < kFixCap: workgroup
+// (c, g) writes chunk c's sum to fix[i][c]; bs_guard_fix_reduce adds them.
+// Entries past kFixCap: workgroup (0, g) adds the chunks itself, in the
+// same order, so a target's result does not depend on its list position.
+template <int MODE, int NF>
+__global__ __launch_bounds__(256) void bs_guard_fix(
+    const double *__restrict__ tx, const double *__restrict__ ty, const double *__restrict__ tz,
+    const Seg64 *__restrict__ seg, const WakeSpan *__restrict__ wk, int n_wk, WakeSpan one,
+    double d, int n, double co, const uint32_t *__restrict__ list, const int *__restrict__ count,
+    double *__restrict__ fix, double *__restrict__ ux, double *__restrict__ uy,
+    double *__restrict__ uz) {
+    __shared__ double red[3][256];
+    const int64_t cnt = *count;
+    const int c = (int)blockIdx.x;
+    for (int64_t i = blockIdx.y; i < cnt; i += gridDim.y) {
+        const int64_t t = list[i];
+        const WakeSpan w = wk ? wk[find_span(wk, n_wk, t)] : one;
+        const Seg64 *s = seg + w.s_off;
+        const double px = tx[t], py = ty[t], pz = tz[t];
+        if (i < kFixCap) {
+            const double3 r = fix_chunk<MODE, NF>(s, w.ns, c, px, py, pz, d, n, co, red);
+            if (threadIdx.x == 0) {
+                double *o = fix + (i * kFixSplit + c) * 3;
+                o[0] = r.x;
+                o[1] = r.y;
+                o[2] = r.z;
+            }
+        } else if (c == 0) {
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            for (int q = 0; q < kFixSplit; ++q) {
+                const double3 r = fix_chunk<MODE, NF>(s, w.ns, q, px, py, pz, d, n, co, red);
+                sx += r.x;
+                sy += r.y;
+                sz += r.z;
+            }
+            if (threadIdx.x == 0) {
+                ux[t] = sx;
+                uy[t] = sy;
+                uz[t] = sz;
+            }
+        }
+    }
+}
+
+__global__ void bs_guard_fix_reduce(const uint32_t *__restrict__ list,
+                                    const int *__restrict__ count,
+                                    const double *__restrict__ fix, double *__restrict__ ux,
+                                    double *__restrict__ uy, double *__restrict__ uz) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t cnt = *count;
+    if (i >= cnt || i >= kFixCap) return;
+    const int64_t t = list[i];
+    const double *o = fix + i * kFixSplit * 3;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int q = 0; q < kFixSplit; ++q, o += 3) {
+        sx += o[0];
+        sy += o[1];
+        sz += o[2];
+    }
+    ux[t] = sx;
+    uy[t] = sy;
+    uz[t] = sz;
+}
+
+__global__ void guard_reset(int *count) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *count = 0;
 }
 
 __global__ void bs_reduce_split(const double *__restrict__ part, int64_t nt,
@@ -317,15 +458,104 @@ template <int CORE, int SEM, bool CO>
 static void fp32c_dispatch(dim3 grid, int block, hipStream_t st, const double *tx,
                            const double *ty, const double *tz, int64_t nt,
                            const Seg32 *s32, const Seg64 *s64, int64_t npad,
-                           int64_t chunk, const Fp32Params &P, double *out, int64_t stride) {
+                           int64_t chunk, const Fp32Params &P, double *out, float *est,
+                           int64_t stride) {
     const size_t lds = (size_t)(block / 64) * sizeof(NearLds);
     bs_fp32c<CORE, SEM, CO><<<grid, block, lds, st>>>(tx, ty, tz, nt, s32, s64, npad, chunk, P, out,
-                                                      stride);
+                                                      est, stride);
+}
+
+static inline size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t guard_bytes(int64_t n_est, int64_t nt) {
+    return al256((size_t)n_est * sizeof(float)) + al256((size_t)nt * sizeof(uint32_t)) + 256 +
+           al256((size_t)kFixCap * kFixSplit * 3 * sizeof(double)) + 256;
+}
+
+GuardBufs guard_carve(void *base, int64_t n_est, int64_t nt) {
+    GuardBufs g;
+    char *p = (char *)(((uintptr_t)base + 255) & ~(uintptr_t)255);
+    g.est = (float *)p;
+    p += al256((size_t)n_est * sizeof(float));
+    g.list = (uint32_t *)p;
+    p += al256((size_t)nt * sizeof(uint32_t));
+    g.count = (int *)p;
+    p += 256;
+    g.fix = (double *)p;
+    return g;
+}
+
+hipError_t launch_guard_reset(const GuardBufs &g, hipStream_t st) {
+    guard_reset<<<1, 64, 0, st>>>(g.count);
+    return hipGetLastError();
+}
+
+hipError_t launch_guard_fix(const double *t3, int64_t nt_total, const Seg64 *seg,
+                            const WakeSpan *wk, int n_wk, WakeSpan one, const EvalCfg &c,
+                            const GuardBufs &g, double *u3, hipStream_t st) {
+    const double *tx = t3, *ty = t3 + nt_total, *tz = t3 + 2 * nt_total;
+    double *ux = u3, *uy = u3 + nt_total, *uz = u3 + 2 * nt_total;
+    const int n = c.core == 0 ? 0 : c.vat_n;
+    // enough workgroups for a few listed targets at a time on every CU; the
+    // kernel exits at once when the list is empty
+    const dim3 grid(kFixSplit, 64);
+#define WINDS_FIX(M, F)                                                                        \
+    bs_guard_fix<M, F><<<grid, 256, 0, st>>>(tx, ty, tz, seg, wk, n_wk, one, c.d, n, c.co, g.list, \
+                                             g.count, g.fix, ux, uy, uz)
+    if (c.semantics == 0)
+        WINDS_FIX(0, 0);
+    else if (n == 0)
+        WINDS_FIX(1, 0);
+    else if (n == 1)
+        WINDS_FIX(2, 1);
+    else if (n == 2)
+        WINDS_FIX(2, 2);
+    else
+        WINDS_FIX(2, 0);
+#undef WINDS_FIX
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    bs_guard_fix_reduce<<<(unsigned)(kFixCap / 256), 256, 0, st>>>(g.list, g.count, g.fix, ux, uy,
+                                                                   uz);
+    return hipGetLastError();
+}
+
+// Far cut-off parameters of the FP32 kernels.  co < 0 zeroes every term
+// (r1 > co and r2 > co hold for every finite pair): co2 = -1, no band.
+static void set_co(Fp32Params &P, const EvalCfg &c) {
+    const bool has_co = c.semantics == 1 && c.co < 1.0e18;
+    if (!has_co) {
+        P.co2 = __builtin_inff();
+        P.co2_tol = 0.0f;
+    } else if (c.co < 0.0) {
+        P.co2 = -1.0f;
+        P.co2_tol = 0.0f;
+    } else {
+        P.co2 = (float)(c.co * c.co);
+        // r^2 in the FP32 path is within a few 1e-7 of its FP64 value
+        P.co2_tol = P.co2 * 1.0e-5f;
+    }
+}
+
+Fp32Params fp32_params(const EvalCfg &c) {
+    Fp32Params P;
+    const int n = c.core == 0 ? 0 : c.vat_n;
+    set_co(P, c);
+    P.neg_inv_n = n > 0 ? -1.0f / (float)n : 0.0f;
+    P.vat_n = n;
+    // general Vatistas order: h^2n overflows FP64 above 2^(1024/n); there the
+    // reference's term is h^2 * inf^(-1/n) = 0, so such pairs take the FP64
+    // branch (which reproduces it); below 2^(128/n) FP32 cannot reach it
+    P.h2_big = __builtin_inff();
+    if (n >= 9) P.h2_big = (float)(exp2(1023.0 / (double)n) * (1.0 - 1e-6));
+    P.d = c.d;
+    P.co = c.co;
+    return P;
 }
 
 hipError_t launch_fp32c(const double *t3, int64_t nt, const Seg64 *seg,
                         int64_t nseg, const EvalCfg &c, double *u3, Seg32 *s32,
-                        double *part, hipStream_t st) {
+                        double *part, const GuardBufs &g, hipStream_t st) {
     if (nt <= 0) return hipSuccess;
     if (nseg <= 0) return launch_zero(u3, 3 * nt, st);
     const int64_t npad = seg32_padded(nseg);
@@ -342,21 +572,18 @@ hipError_t launch_fp32c(const double *t3, int64_t nt, const Seg64 *seg,
     const int nsplit = (int)((npad + chunk - 1) / chunk);
     dim3 grid(grid_for(nt, block), nsplit);
     const double *tx = t3, *ty = t3 + nt, *tz = t3 + 2 * nt;
-    double *out = nsplit == 1 ? u3 : part;
-    const double cof = c.co;
-    const float co2 = (c.semantics == 1 && cof < 1.0e18) ? (float)(cof * cof) : __builtin_inff();
-    const float ninv = n > 0 ? -1.0f / (float)n : 0.0f;
+    const bool guard = c.guard >= 0.0;
+    double *out = nsplit == 1 && !guard ? u3 : part;
+    float *est = guard ? g.est : nullptr;
     const int kind = c.core == 0 ? 0 : (n == 1 ? 1 : (n == 2 ? 2 : 3));
-    Fp32Params P;
-    P.co2 = co2;
-    // r^2 in the FP32 path is within a few 1e-7 of its FP64 value
-    P.co2_tol = P.co2 * 1.0e-5f;
-    P.neg_inv_n = ninv;
-    P.vat_n = n;
-    P.d = c.d;
-    P.co = c.co;
+    const Fp32Params P = fp32_params(c);
+    if (guard) {
+        e = launch_guard_reset(g, st);
+        if (e != hipSuccess) return e;
+    }
 #define WINDS_FP32C(K, S, C)                                                                \
-    fp32c_dispatch<K, S, C>(grid, block, st, tx, ty, tz, nt, s32, seg, npad, chunk, P, out, nt)
+    fp32c_dispatch<K, S, C>(grid, block, st, tx, ty, tz, nt, s32, seg, npad, chunk, P, out, est, \
+                            nt)
     const bool has_co = c.semantics == 1 && c.co < 1.0e18;
     if (c.semantics == 0) {
         WINDS_FP32C(0, 0, false);
@@ -378,6 +605,13 @@ hipError_t launch_fp32c(const double *t3, int64_t nt, const Seg64 *seg,
 #undef WINDS_FP32C
     e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (guard) {
+        bs_reduce_guard<<<grid_for(nt, 256), 256, 0, st>>>(part, g.est, nt, nsplit, c.guard, u3,
+                                                           u3 + nt, u3 + 2 * nt, g.list, g.count);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        return launch_guard_fix(t3, nt, seg, nullptr, 0, WakeSpan{0, nt, 0, nseg}, c, g, u3, st);
+    }
     if (nsplit > 1) {
         bs_reduce_split<<<grid_for(nt, 256), 256, 0, st>>>(part, nt, nsplit, u3, u3 + nt, u3 + 2 * nt);
         e = hipGetLastError();

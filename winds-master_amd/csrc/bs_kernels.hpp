@@ -42,6 +42,27 @@ constexpr int32_t kSegSkip = 2;     // padding: no term
 static_assert(sizeof(Seg32) == 64, "Seg32 layout");
 static_assert(sizeof(Seg64) == 64, "Seg64 layout");
 
+// Cancellation guard of the FP32 path.  Each target also accumulates
+// a2 = sum over its 16-record tiles of |tile partial|^2 (FP32 terms only).
+// The FP32 error of a target stays below ~4e-7 * sqrt(a2) (measured by
+// emulating the kernel's arithmetic on rotor wakes of 87k-134k targets,
+// Vatistas-2 and cut-off cores), so a target with sqrt(a2) > kappa |u| has
+// FP32 terms that cancel to a small result, and its sum is evaluated again
+// in FP64 over every segment.  kappa = 1 keeps the FP32 error below ~4e-7
+// |u| and selects ~0.1% of the targets of those wakes.
+constexpr double kGuardKappa = 1.0;
+// Targets re-evaluated with the segment range split across kFixSplit
+// workgroups (partials reduced in a fixed order); targets past kFixCap in
+// the list are summed by one workgroup each.
+constexpr int kFixSplit = 16;
+constexpr int64_t kFixCap = 4096;
+
+// Targets [t_off, t_off + nt) of one wake and its FP64 records
+// [s_off, s_off + ns).
+struct WakeSpan {
+    int64_t t_off, nt, s_off, ns;
+};
+
 struct EvalCfg {
     int core = 0;        // 0 cut-off, 1 Vatistas
     int vat_n = 2;
@@ -55,7 +76,21 @@ struct EvalCfg {
     int split = 0;
     int64_t seg_col = 0;
     int64_t seg_blk = 0;
+    double guard = kGuardKappa;  // FP32C guard threshold kappa; < 0: off
 };
+
+// Device buffers of the FP32 guard: n_est a2 partials (one per target and
+// partial sum), the list of targets to re-evaluate (room for nt) and its
+// length, and the partials of the split re-evaluation
+// ([kFixCap][kFixSplit][3]).
+struct GuardBufs {
+    float *est = nullptr;
+    uint32_t *list = nullptr;
+    int *count = nullptr;
+    double *fix = nullptr;
+};
+size_t guard_bytes(int64_t n_est, int64_t nt);
+GuardBufs guard_carve(void *base, int64_t n_est, int64_t nt);
 
 int64_t seg32_padded(int64_t nseg);
 
@@ -64,9 +99,19 @@ hipError_t launch_zero(double *p, int64_t n, hipStream_t st);
 // segment range across workgroups; otherwise one sequential sum per target.
 hipError_t launch_fp64(const double *t3, int64_t nt, const Seg64 *seg, int64_t nseg,
                        const EvalCfg &c, double *u3, hipStream_t st, double *part = nullptr);
+// FP32 compensated sum.  part holds max(split, 1) * 3 * nt doubles; with
+// c.guard >= 0, g carries the guard's buffers (split = the split in c).
 hipError_t launch_fp32c(const double *t3, int64_t nt, const Seg64 *seg, int64_t nseg,
                         const EvalCfg &c, double *u3, Seg32 *s32, double *part,
-                        hipStream_t st);
+                        const GuardBufs &g, hipStream_t st);
+// FP64 re-evaluation of the targets in g.list (count at g.count): the
+// guard's second pass.  wk (device, n_wk entries) locates each target's
+// wake; with wk null the single wake `one` holds every target.
+hipError_t launch_guard_fix(const double *t3, int64_t nt_total, const Seg64 *seg,
+                            const WakeSpan *wk, int n_wk, WakeSpan one, const EvalCfg &c,
+                            const GuardBufs &g, double *u3, hipStream_t st);
+// Zero the guard's counter (a kernel, so that graph capture records no memset).
+hipError_t launch_guard_reset(const GuardBufs &g, hipStream_t st);
 hipError_t launch_pack_segments(const double *F1, const double *F2, const int64_t *fd,
                                 const double *gamma, const double *rc, Seg64 *out,
                                 hipStream_t st);

@@ -76,14 +76,28 @@ __device__ __forceinline__ double inv_root(double y, int n) {
 }
 
 // y^(-1/n) for n >= 1 without pow: an FP32 first guess and three Newton
-// steps on r^-n = y in FP64 (quadratic convergence from ~1e-6).
+// steps on r^-n = y in FP64 (quadratic convergence from ~1e-6).  The guess
+// is taken from y = m 2^e (m in [0.5, 1)) as 2^k * 2^(f - log2(m)/n) with
+// -e/n = k + f (k integer, f in [0, 1)), so that it stays inside the FP32
+// range for every FP64 y; the residual 1 - y r^n is formed as a running
+// product from y, which moves monotonically towards 1 and cannot overflow.
+// pow's special values are kept: y = 0 -> +inf, y = +inf -> 0, y < 0 or
+// NaN -> NaN (BiotSavart.m's (rc^n + Cnu^n)^(-1/n) when Cnu^n overflows is
+// inf^(-1/n) = 0).
 __device__ __forceinline__ double inv_root_newton(double y, int n) {
+    if (!(y > 0.0)) return y == 0.0 ? __builtin_inf() : __builtin_nan("");
+    if (__builtin_isinf(y)) return 0.0;
     const double inv_n = 1.0 / (double)n;
-    double r = (double)__builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)y) * (float)(-inv_n));
+    int e;
+    const double m = frexp(y, &e);
+    const double q = -(double)e * inv_n;
+    const double k = floor(q);
+    const float ex = (float)(q - k) - __builtin_amdgcn_logf((float)m) * (float)inv_n;
+    double r = ldexp((double)__builtin_amdgcn_exp2f(ex), (int)k);
     for (int it = 0; it < 3; ++it) {
-        double rn = r;
-        for (int i = 1; i < n; ++i) rn = rn * r;
-        r = fma(r * fma(-y, rn, 1.0), inv_n, r);
+        double t = y;
+        for (int i = 0; i < n; ++i) t = t * r;
+        r = fma(r * (1.0 - t), inv_n, r);
     }
     return r;
 }

@@ -69,6 +69,11 @@ std::string run(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
     int64_t f2d[4];
     if (!dims4(F2, f2d) || std::memcmp(fd, f2d, sizeof(fd)) != 0)
         return "F1 and F2 must have the same dims.";
+    // the engine reads ns*nt*nb values of gamma and of rc (one per segment)
+    const size_t nf = (size_t)(fd[0] * fd[2] * fd[3]);
+    if (mxGetNumberOfElements(G) != nf) return "gamma must hold one value per segment.";
+    const size_t nrc = mxGetNumberOfElements(RC);
+    if (nrc != 0 && nrc != nf) return "rc must be empty or hold one value per segment.";
     double hw[3] = {0.0, 0.0, 0.0};
     const size_t nhw = mxGetNumberOfElements(HW);
     for (size_t i = 0; i < 3 && i < nhw; ++i) hw[i] = mxGetPr(HW)[i];
@@ -83,7 +88,8 @@ std::string run(int nlhs, mxArray *plhs[], int nrhs, const mxArray *prhs[]) {
         return "out of host memory";
     }
     const int rc = winds_bs_eval(mxGetPr(F1), mxGetPr(F2), fd, mxGetPr(P), pd, mxGetPr(G),
-                                 mxGetPr(RC), mxGetPr(D)[0], cmod.c_str(), mxGetPr(CO)[0],
+                                 nrc ? mxGetPr(RC) : nullptr, mxGetPr(D)[0], cmod.c_str(),
+                                 mxGetPr(CO)[0],
                                  type.c_str(), gpu.c_str(), hw, mxGetPr(U), mxGetPr(L));
     if (rc != WINDS_OK) {
         mxDestroyArray(U);

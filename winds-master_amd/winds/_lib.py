@@ -21,7 +21,7 @@ EXPORTS = (
     "winds_pack_targets_dev", "winds_unpack_uind_dev", "winds_segment_length_dev",
     "winds_device_count", "winds_batch_create", "winds_batch_destroy",
     "winds_batch_workspace_bytes", "winds_batch_eval_dev", "winds_batch_autotune",
-    "winds_batch_info",
+    "winds_batch_info", "winds_bs_guard_count", "winds_batch_guard_count",
 )
 
 
@@ -37,7 +37,7 @@ class BsOpts(C.Structure):
         ("precision", C.c_int32), ("d", C.c_double), ("co", C.c_double),
         ("near_factor", C.c_double), ("near_len_frac", C.c_double),
         ("block", C.c_int32), ("split", C.c_int32), ("seg_col", C.c_int32),
-        ("seg_blk", C.c_int32),
+        ("seg_blk", C.c_int32), ("guard_kappa", C.c_double),
     ]
 
 
@@ -87,6 +87,9 @@ def lib():
     L.winds_unpack_uind_dev.restype = C.c_int
     L.winds_segment_length_dev.argtypes = [_P, C.c_int64, C.c_int, _P, _P]
     L.winds_segment_length_dev.restype = C.c_int
+    L.winds_bs_guard_count.argtypes = [_P, C.c_int64, C.c_int64, C.POINTER(BsOpts),
+                                       C.POINTER(C.c_int64), _P]
+    L.winds_bs_guard_count.restype = C.c_int
     _lib = L
     return L
 
@@ -99,12 +102,14 @@ def check(code: int) -> None:
 def opts(core: int = CORE_CUTOFF, vat_n: int = 2, semantics: int = SEM_MEX,
          precision: int = PREC_FP64, d: float = 1e-4, co: float = math.inf,
          near_factor: float = 40.0, near_len_frac: float = 0.51, block: int = 0,
-         split: int = 0, seg_col: int = 0, seg_blk: int = 0) -> BsOpts:
+         split: int = 0, seg_col: int = 0, seg_blk: int = 0,
+         guard_kappa: float = 0.0) -> BsOpts:
     o = BsOpts()
     lib().winds_bs_opts_default(C.byref(o))
     o.core, o.vat_n, o.semantics, o.precision = core, vat_n, semantics, precision
     o.d, o.co, o.near_factor, o.near_len_frac = d, co, near_factor, near_len_frac
     o.block, o.split, o.seg_col, o.seg_blk = block, split, seg_col, seg_blk
+    o.guard_kappa = guard_kappa
     return o
 
 

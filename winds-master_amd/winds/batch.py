@@ -41,6 +41,8 @@ def _bind(L):
     L.winds_batch_autotune.restype = C.c_int
     L.winds_batch_info.argtypes = [P, C.POINTER(C.c_int64)]
     L.winds_batch_info.restype = C.c_int
+    L.winds_batch_guard_count.argtypes = [P, P, C.POINTER(C.c_int64), P]
+    L.winds_batch_guard_count.restype = C.c_int
     L._winds_batch_bound = True
     return L
 
@@ -122,6 +124,16 @@ class Batch:
             C.c_void_p(self.out.data_ptr()), C.c_void_p(st), C.byref(best), C.byref(ms))
         _lib.check(code)
         return best.as_dict(), float(ms.value)
+
+    def guard_count(self, stream: torch.cuda.Stream | None = None) -> int:
+        """Targets the FP32C guard re-evaluated in FP64 in the last evaluation."""
+        if self._ws is None:
+            return 0
+        st = (stream or torch.cuda.current_stream(self.device)).cuda_stream
+        n = C.c_int64()
+        _lib.check(self.L.winds_batch_guard_count(self._plan, C.c_void_p(self._ws.data_ptr()),
+                                                  C.byref(n), C.c_void_p(st)))
+        return int(n.value)
 
     def split(self, u: torch.Tensor | None = None):
         """Per-wake [3, nt] views of a batch result."""

@@ -47,8 +47,21 @@ class Evaluator:
             raise ValueError("out must be a contiguous float64 [3, n] tensor")
         ws = self.workspace(nt, ns, o)
         st = (stream or torch.cuda.current_stream(self.device)).cuda_stream
+        self._last = (nt, ns, o)
         code = _lib.lib().winds_bs_eval_dev(
             C.c_void_p(targets.data_ptr()), nt, C.c_void_p(segs.data_ptr()), ns, C.byref(o),
             C.c_void_p(out.data_ptr()), C.c_void_p(ws.data_ptr()), ws.numel(), C.c_void_p(st))
         _lib.check(code)
         return out
+
+    def guard_count(self, stream: torch.cuda.Stream | None = None) -> int:
+        """Targets the FP32C guard re-evaluated in FP64 in the last eval()."""
+        last = getattr(self, "_last", None)
+        if last is None or self._ws is None:
+            return 0
+        nt, ns, o = last
+        st = (stream or torch.cuda.current_stream(self.device)).cuda_stream
+        n = C.c_int64()
+        _lib.check(_lib.lib().winds_bs_guard_count(C.c_void_p(self._ws.data_ptr()), nt, ns,
+                                                   C.byref(o), C.byref(n), C.c_void_p(st)))
+        return int(n.value)
