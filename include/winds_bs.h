@@ -135,6 +135,23 @@ int winds_bs_eval_dev(const double *targets, int64_t n_targets,
                       size_t workspace_bytes, void *stream);
 
 /*
+ * winds_bs_eval_dev without a workspace argument and without any device
+ * allocation: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The workspace is a 64 MiB device array
+ * inside the library (one per GPU, held until the process ends); the FP32C
+ * segment-range split is lowered until the evaluation fits in it, and an
+ * evaluation that does not fit even unsplit (about 1M segments, or 2.4M
+ * targets at 1 split) returns WINDS_E_NOMEM before launching anything.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (different streams, different threads) race and must be serialised by the
+ * caller.  Targets, segments and uind are as for winds_bs_eval_dev.
+ */
+int winds_bs_eval_dev_noalloc(const double *targets, int64_t n_targets,
+                              const double *segments, int64_t n_segments,
+                              const winds_bs_opts *o, double *uind, void *stream);
+
+/*
  * Number of targets the FP32C cancellation guard evaluated again in FP64 in
  * the last evaluation that used this workspace with these sizes and options
  * (0 for FP64 or with the guard off).  Synchronises stream.  No counterpart

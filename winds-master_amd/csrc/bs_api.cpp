@@ -299,6 +299,75 @@ int winds_bs_eval_dev(const double *targets, int64_t n_targets, const double *se
                          workspace, workspace_bytes, (hipStream_t)stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Workspace of winds_bs_eval_dev_noalloc: a file-scope device array (one
+// copy per GPU, held until the process ends).  In host code `scratch` names
+// a host-side stand-in, not device memory: its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDev] = {};
+
+int scratch_address(void **out) {
+    int dev = 0;
+    WINDS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDev) return fail(WINDS_E_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(WINDS_E_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                         hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return WINDS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int winds_bs_eval_dev_noalloc(const double *targets, int64_t n_targets, const double *segments,
+                              int64_t n_segments, const winds_bs_opts *o, double *uind,
+                              void *stream) {
+    g_err.clear();
+    winds::EvalCfg c;
+    int rc = to_cfg(o, c);
+    if (rc) return rc;
+    if (n_targets < 0 || n_segments < 0) return fail(WINDS_E_DIMS, "negative size");
+    if (n_targets > 0 && (!targets || !uind)) return fail(WINDS_E_ARG, "null target/output pointer");
+    if (n_segments > 0 && !segments) return fail(WINDS_E_ARG, "null segment pointer");
+    if (n_targets > (int64_t)1 << 31 || n_segments > (int64_t)1 << 31)
+        return fail(WINDS_E_DIMS, "more than 2^31 targets or segments");
+    // the tuned split of the FP32C segment range, or the largest that fits
+    // in the scratch array; a split asked for explicitly must fit as it is
+    int split = 1;
+    size_t need = ws_bytes(n_targets, n_segments, c, &split);
+    if (need > kScratchBytes && c.precision == WINDS_PREC_FP32C && o->split == 0) {
+        while (split > 1 && need > kScratchBytes) {
+            c.split = --split;
+            need = ws_bytes(n_targets, n_segments, c, &split);
+        }
+    }
+    if (need > kScratchBytes)
+        return fail(WINDS_E_NOMEM, "evaluation needs " + std::to_string(need) +
+                                       " bytes of workspace; the no-allocation entry point has " +
+                                       std::to_string(kScratchBytes));
+    void *ws = nullptr;
+    rc = scratch_address(&ws);
+    if (rc) return rc;
+    if (c.precision == WINDS_PREC_FP32C) c.split = split;
+    return eval_dev_impl(targets, n_targets, (const winds::Seg64 *)segments, n_segments, c, uind,
+                         ws, kScratchBytes, (hipStream_t)stream);
+}
+
 int winds_bs_guard_count(const void *workspace, int64_t n_targets, int64_t n_segments,
                          const winds_bs_opts *o, int64_t *count, void *stream) {
     g_err.clear();

@@ -22,6 +22,7 @@ EXPORTS = (
     "winds_device_count", "winds_batch_create", "winds_batch_destroy",
     "winds_batch_workspace_bytes", "winds_batch_eval_dev", "winds_batch_autotune",
     "winds_batch_info", "winds_bs_guard_count", "winds_batch_guard_count",
+    "winds_bs_eval_dev_noalloc",
 )
 
 
@@ -90,6 +91,9 @@ def lib():
     L.winds_bs_guard_count.argtypes = [_P, C.c_int64, C.c_int64, C.POINTER(BsOpts),
                                        C.POINTER(C.c_int64), _P]
     L.winds_bs_guard_count.restype = C.c_int
+    L.winds_bs_eval_dev_noalloc.argtypes = [_P, C.c_int64, _P, C.c_int64, C.POINTER(BsOpts), _P,
+                                            _P]
+    L.winds_bs_eval_dev_noalloc.restype = C.c_int
     _lib = L
     return L
 
