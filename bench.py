@@ -40,20 +40,60 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def host_cores() -> int:
+def host_info() -> dict:
+    """Host cores this process may use and the CPU model string."""
     try:
-        n = len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        aff = os.cpu_count() or 1
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    # OMP_NUM_THREADS, where set, is the host share given to this process
+    # (16 per GPU on the GPU hosts); otherwise every core of the affinity mask
+    threads = aff
+    if omp and omp.isdigit() and int(omp) > 0:
+        threads = min(aff, int(omp))
+    return {"threads": max(1, threads), "affinity_cores": aff, "os_cpu_count": os.cpu_count(),
+            "omp_num_threads": omp, "cpu_model": model}
 
 
 def cpu_baseline(wake, budget_s: float = 12.0) -> dict:
-    """The oracle's BiotSavart_nogpu restatement on a bounded target sample."""
+    """The oracle's restatement of BiotSavart_nogpu (BiotSavartMex.cu:268-282,
+    FP64, cut-off core): config 1 (3,888 x 7,506) in full on one thread and
+    on all the host cores this process has, and a bounded sample of the
+    single large wake (the GPU workload) on all of them."""
     import numpy as np
     from oracle import bs as O
-    from winds import layout
+    from winds import gen, layout
 
+    hi = host_info()
+    threads = hi["threads"]
+    res = {}
+    # config 1, whole evaluation (shed + trailing segments, all nodes)
+    c1 = gen.config1()
+    r1 = layout.wake_segments(c1)
+    F1c = np.asfortranarray(r1[:, 0:3].reshape(-1, 3, 1, 1))
+    F2c = np.asfortranarray(r1[:, 3:6].reshape(-1, 3, 1, 1))
+    g1 = r1[:, 6].copy()
+    inter1 = c1.n_targets * r1.shape[0]
+    for label, th in (("config1_all", threads), ("config1_one", 1)):
+        O.bs_mex(F1c, F2c, c1.domain, g1, 1e-4, threads=th)
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            O.bs_mex(F1c, F2c, c1.domain, g1, 1e-4, threads=th)
+            reps += 1
+            if time.perf_counter() - t0 > 2.0:
+                break
+        dt = (time.perf_counter() - t0) / reps
+        res[label] = dict(rate=inter1 / dt / 1e9, ms=dt * 1e3, threads=th)
+    # the single large wake: first n targets x all segments
     recs = layout.wake_segments(wake)
     F1 = np.asfortranarray(recs[:, 0:3].reshape(-1, 3, 1, 1))
     F2 = np.asfortranarray(recs[:, 3:6].reshape(-1, 3, 1, 1))
@@ -61,29 +101,31 @@ def cpu_baseline(wake, budget_s: float = 12.0) -> dict:
     P = wake.domain
     nseg = recs.shape[0]
     nt = wake.n_targets
-    threads = host_cores()
-    out = {}
-    for label, th, frac in (("all", threads, 0.75), ("one", 1, 0.25)):
-        n = 16 * th if th > 1 else 16
-        t0 = time.perf_counter()
-        O.bs_mex_range(F1, F2, P, g, 1e-4, 0, n, threads=th)
-        dt = time.perf_counter() - t0
-        n2 = int(min(nt, max(n, n * budget_s * frac / max(dt, 1e-3))))
-        t0 = time.perf_counter()
-        O.bs_mex_range(F1, F2, P, g, 1e-4, 0, n2, threads=th)
-        dt = time.perf_counter() - t0
-        out[label] = dict(rate=n2 * nseg / dt / 1e9, targets=n2, seconds=dt, threads=th)
-    a = out["all"]
+    n = 16 * threads
+    t0 = time.perf_counter()
+    O.bs_mex_range(F1, F2, P, g, 1e-4, 0, n, threads=threads)
+    dt = time.perf_counter() - t0
+    n2 = int(min(nt, max(n, n * budget_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    O.bs_mex_range(F1, F2, P, g, 1e-4, 0, n2, threads=threads)
+    dt = time.perf_counter() - t0
+    rate = n2 * nseg / dt / 1e9
+    a1, o1 = res["config1_all"], res["config1_one"]
     return {
-        "value": round(a["rate"], 4),
+        "value": round(rate, 4),
         "unit": "G interactions/s",
-        "cores": a["threads"],
+        "cores": threads,
         "kind": "port",
-        "sample": (f"first {a['targets']} of {nt} targets x all {nseg} segments, cut-off core "
-                   f"(BiotSavart_nogpu arithmetic, FP64), {a['seconds']:.1f} s on "
-                   f"{a['threads']} OpenMP threads; 1 thread: "
-                   f"{out['one']['rate']:.4f} G int/s on {out['one']['targets']} targets"),
-        "value_1thread": round(out["one"]["rate"], 4),
+        "sample": (f"first {n2} of {nt} targets x all {nseg} segments of the single large wake, "
+                   f"cut-off core (BiotSavart_nogpu arithmetic, FP64, the oracle's C "
+                   f"restatement), {dt:.1f} s on {threads} OpenMP threads ({hi['cpu_model']}); "
+                   f"config 1 in full: {a1['ms']:.1f} ms on {threads} threads, "
+                   f"{o1['ms']:.1f} ms on 1 thread"),
+        "host": hi,
+        "config1_full_ms": {"threads": threads, "all_cores": round(a1["ms"], 2),
+                            "one_thread": round(o1["ms"], 2),
+                            "interactions": inter1},
+        "config1_G_int_s": {"all_cores": round(a1["rate"], 4), "one_thread": round(o1["rate"], 4)},
     }
 
 
@@ -118,28 +160,41 @@ def batch_bench(o32, o64, steps: int) -> dict:
     ms = e0.elapsed_time(e1) / n
     rate = info["interactions"] / (ms * 1e-3) / 1e9
     log(f"batch: {ms:.1f} ms per batch evaluation, {rate:.1f} G int/s")
-    # error of four wakes against the FP64 engine
-    ev = Evaluator(0)
-    parts = b.split()
-    worst = {"max_rel": 0.0, "rms_rel": 0.0, "normwise": 0.0}
-    for i in (0, 17, 40, 63):
-        w = wakes[i]
-        tg = torch.from_numpy(layout.wake_targets(w)).cuda().contiguous()
-        sg = torch.from_numpy(layout.wake_segments(w)).cuda().contiguous()
-        ref = ev.eval(tg, sg, o64).cpu().numpy()
-        u = parts[i].cpu().numpy()
+    n_fix = b.guard_count()
+    u32 = b.out.clone()
+    # every wake against its own FP64 evaluation: the FP64 batch (one launch,
+    # each wake bitwise equal to its single-wake FP64 evaluation)
+    b64 = Batch(wakes, o64)
+    b64.eval(use_graph=False)
+    torch.cuda.synchronize()
+    f0 = time.perf_counter()
+    u64 = b64.eval(use_graph=False)
+    torch.cuda.synchronize()
+    ms64 = (time.perf_counter() - f0) * 1e3
+    worst = {"max_rel": 0.0, "rms_rel": 0.0, "normwise": 0.0, "wake_of_max": -1}
+    for i, (a, r) in enumerate(zip(b64.split(u32), b64.split(u64))):
+        ref = r.cpu().numpy()
+        u = a.cpu().numpy()
         nrm = np.linalg.norm(ref, axis=0)
         dif = np.linalg.norm(u - ref, axis=0)
         m = nrm >= 1e-6 * nrm.max()
         e = dif[m] / nrm[m]
-        worst["max_rel"] = max(worst["max_rel"], float(e.max()))
+        if float(e.max()) > worst["max_rel"]:
+            worst["max_rel"] = float(e.max())
+            worst["wake_of_max"] = i
         worst["rms_rel"] = max(worst["rms_rel"], float(np.sqrt(np.mean(e * e))))
         worst["normwise"] = max(worst["normwise"], float(dif.max() / nrm.max()))
-    log(f"batch error (4 wakes vs FP64): {worst}")
+    log(f"batch error (all {len(wakes)} wakes vs FP64): {worst}; guard re-evaluated {n_fix} "
+        f"targets; FP64 batch {ms64:.0f} ms")
+    del b64
     return {"value": round(rate, 2), "unit": "G interactions/s", "ms_per_batch": round(ms, 3),
             "wakes": len(wakes), "targets": info["targets"], "segments": info["segments"],
             "interactions": info["interactions"], "work_items": info["items"],
-            "tune": tune, "graph": True, "error_4_wakes_vs_fp64": worst}
+            "tune": tune, "graph": True,
+            "error_all_wakes_vs_fp64": worst,
+            "guard_fp64_targets": n_fix,
+            "fp64_batch_ms": round(ms64, 1),
+            "fp64_batch_G_int_s": round(info["interactions"] / (ms64 * 1e-3) / 1e9, 2)}
 
 
 def march_opts(prec):
@@ -433,11 +488,16 @@ def main():
                      precision=winds.PREC_FP64, co=math.inf)
     out = torch.empty((3, nt), dtype=torch.float64, device=dev)
 
-    # FP64 engine result (reference for the error), checked on a sample
-    t0 = time.perf_counter()
+    # FP64 engine result (reference for the error), checked on a sample;
+    # the second evaluation is timed (the first loads the code)
     u64 = ev.eval(tg, sg, o64).clone()
     torch.cuda.synchronize()
-    t64 = time.perf_counter() - t0
+    f0, f1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    f0.record(torch.cuda.current_stream(dev))
+    ev.eval(tg, sg, o64, out=u64)
+    f1.record(torch.cuda.current_stream(dev))
+    torch.cuda.synchronize()
+    t64 = f0.elapsed_time(f1) * 1e-3
     log(f"fp64 evaluation: {t64 * 1e3:.1f} ms")
     rng = np.random.default_rng(11)
     sample = np.sort(rng.choice(nt, size=48, replace=False))
@@ -477,7 +537,22 @@ def main():
     err = {"max_rel": float(e.max()), "rms_rel": float(np.sqrt(np.mean(e * e))),
            "normwise": float(dif.max() / nrm.max()), "vs": "FP64 engine (BiotSavart.m arithmetic)",
            "fp64_vs_cpu_oracle_normwise": fp64_vs_oracle, "targets": int(m.sum())}
+    n_fix = ev.guard_count()
+    err["guard_fp64_targets"] = n_fix
     log(f"fp32c error vs fp64: {err}")
+    # the same evaluation with the cancellation guard off (its cost)
+    og = winds.opts(core=winds.CORE_VATISTAS, vat_n=2, semantics=winds.SEM_MATLAB,
+                    precision=winds.PREC_FP32C, co=math.inf, guard_kappa=-1.0)
+    ev.eval(tg, sg, og, out=out)
+    torch.cuda.synchronize()
+    g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    g0.record(st)
+    for _ in range(max(3, args.steps // 4)):
+        ev.eval(tg, sg, og, out=out)
+    g1.record(st)
+    torch.cuda.synchronize()
+    ms_noguard = g0.elapsed_time(g1) / max(3, args.steps // 4)
+    log(f"fp32c without the guard: {ms_noguard:.3f} ms")
 
     # cut-off (MEX semantics) rate, same workload
     oc = winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, precision=winds.PREC_FP32C,
@@ -546,6 +621,7 @@ def main():
                        "ms_per_step": round(ms_c, 4),
                        "tflops": round(inter * FLOP_CUTOFF / (ms_c * 1e-3) / 1e12, 2)},
         "fp64_ms_per_step": round(t64 * 1e3, 2),
+        "ms_per_step_guard_off": round(ms_noguard, 4),
         "drop_in_call_ms_host_arrays": round(abi_ms, 2),
         "batch_64_wakes": batch,
         "far_wake_surrogate": surr_rep,
@@ -564,7 +640,10 @@ def main():
     }
     if not args.no_cpu:
         log("cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline(wake)
+        cb = cpu_baseline(wake)
+        # same core and arithmetic as the CPU path: the GPU cut-off rate
+        cb["gpu_cutoff_over_cpu_all_cores"] = round(rate_c / cb["value"], 1)
+        line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 
 

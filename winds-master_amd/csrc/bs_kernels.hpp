@@ -54,8 +54,8 @@ constexpr double kGuardKappa = 1.0;
 // Targets re-evaluated with the segment range split across kFixSplit
 // workgroups (partials reduced in a fixed order); targets past kFixCap in
 // the list are summed by one workgroup each.
-constexpr int kFixSplit = 16;
-constexpr int64_t kFixCap = 4096;
+constexpr int kFixSplit = 64;
+constexpr int64_t kFixCap = 2048;
 
 // Targets [t_off, t_off + nt) of one wake and its FP64 records
 // [s_off, s_off + ns).
