@@ -303,33 +303,36 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
 
 
 def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train_steps: int,
-                check_every: int = 72):
+                check_every: int = 72, fp64_revs: int = 2):
     """Free-wake time march of a 3-blade, 31-node rotor at 5 deg steps with
     PCC (2 wake evaluations per step), an FP64 Kutta-Joukowski solve and a
-    1,440-age wake:
+    1,440-age wake (~10 rotor diameters):
     (1) a cold start of dev_revs revolutions with FP32 compensated wake sums
         (direct-sum snapshots of its far wake feed the surrogate's training);
-    (2) step time at the full length from that state: FP32, FP64, surrogate;
-    (3) `revs` more revolutions from that state, FP32 vs surrogate thrust/power;
-    (4) the first `revs` revolutions of the cold start, FP32 vs FP64.
+    (2) step time at the full length from that state: FP32, FP64, surrogate
+        (network), surrogate with the direct far sum instead of the network;
+    (3) `revs` more revolutions from that state: FP32 and surrogate, and the
+        first fp64_revs of them in FP64 as the reference for both.
     Returns (march report, surrogate report)."""
     import numpy as np
 
     import winds
-    from winds import rotor as rmod
+    from winds import cases
     from winds import surrogate as S
     from winds.march import March
 
-    rot = rmod.synthetic_rotor(nb=3, nst=31)
+    per_rev = 72
+    steps = revs * per_rev
+    dev_steps = dev_revs * per_rev
+    tb = cases.tables(cases.synthetic_rotor(nb=3, nst=31, n_times=dev_steps + steps + 16))
     sc = S.MarchSurrogate(check_every=check_every)
-    per_rev = int(round(360 / rot.dpsi_deg))
-    out = {"rotor": "3 blades x 31 nodes, R 63 m, TSR 7, U 11.4 m/s, 5 deg steps, PCC",
+    out = {"rotor": "3 blades x 31 nodes, R 63 m, TSR 7, U 11.4 m/s, 5 deg steps, PCC, "
+                    "kinematics.m/velocity.m tables, BEM start",
            "max_ages": 1440, "develop_revs": dev_revs}
     o32 = march_opts(winds.PREC_FP32C)
     # (1) develop
     t0 = time.perf_counter()
-    dev_steps = dev_revs * per_rev
-    m = March(rot, 1440, o32, integrator="pcc", max_steps=dev_steps + 8)
+    m = March(tb, 1440, o32, integrator="pcc")
     first = max(0, dev_steps - 10 * per_rev)
     m.step(first)
     m.sync()
@@ -343,7 +346,6 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
         m.sync()
     dev_s = time.perf_counter() - t0
     log(f"march cold start {dev_revs} revs: {dev_s:.1f} s")
-    cold32 = m.perf()
     inf = m.info()
     nodes, _, _ = m.state()
     A, Ag = inf["ages"], inf["ages_solved"]
@@ -357,19 +359,24 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
         surr_rep, net, in_norm, out_norm = surrogate_bench(n_samples, train_steps, msamp)
         net_args = (net, in_norm, out_norm)
 
-    def fork(prec, with_surr, max_steps=64):
-        f = March(rot, 1440, march_opts(prec), integrator="pcc", max_steps=max_steps)
+    def fork(prec, surr_mode=None):
+        f = March(tb, 1440, march_opts(prec), integrator="pcc")
         f.set_state(*state)
-        if with_surr:
-            f.set_surrogate(sc, *net_args)
+        if surr_mode is not None:
+            cfg = S.MarchSurrogate(mode=surr_mode, check_every=check_every)
+            if surr_mode == 0:
+                f.set_surrogate(cfg, *net_args)
+            else:
+                f.set_surrogate(cfg)
         return f
 
     # (2) step time at the full length
-    runs = [("fp32c", winds.PREC_FP32C, 10, False), ("fp64", winds.PREC_FP64, 2, False)]
+    runs = [("fp32c", winds.PREC_FP32C, 10, None), ("fp64", winds.PREC_FP64, 2, None)]
     if surrogate:
-        runs.append(("surrogate", winds.PREC_FP32C, 2 * sc.check_every, True))
-    for name, prec, nsteps, ws in runs:
-        f = fork(prec, ws, max_steps=nsteps + 8)
+        runs += [("surrogate", winds.PREC_FP32C, 2 * check_every, 0),
+                 ("surrogate_direct_far", winds.PREC_FP32C, 2 * check_every, 1)]
+    for name, prec, nsteps, mode in runs:
+        f = fork(prec, mode)
         f.step(1)
         f.sync()
         t0 = time.perf_counter()
@@ -378,62 +385,62 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
         ms = (time.perf_counter() - t0) / nsteps * 1e3
         out[f"step_ms_{name}"] = round(ms, 2)
         log(f"march step at 1440 ages, {name}: {ms:.1f} ms")
-        if ws:
-            out["surrogate_stats_steady"] = f.surr_stats()
-            out["surrogate_speedup_vs_fp32c"] = round(out["step_ms_fp32c"] / ms, 2)
-            log(f"  surrogate {out['surrogate_speedup_vs_fp32c']}x vs fp32c; "
-                f"{out['surrogate_stats_steady']}")
+        if mode is not None:
+            out[f"{name}_stats_steady"] = f.surr_stats()
+            out[f"{name}_speedup_vs_fp32c"] = round(out["step_ms_fp32c"] / ms, 2)
+            log(f"  {name} {out[f'{name}_speedup_vs_fp32c']}x vs fp32c; "
+                f"{out[f'{name}_stats_steady']}")
         del f
-    # (3) revs more revolutions: plain FP32 vs surrogate
-    steps = revs * per_rev
+    # (3) revs more revolutions from the developed state
+    hist = {}
+    plan = [("fp32c", winds.PREC_FP32C, None, steps)]
     if surrogate:
-        hist = {}
-        for name, ws in (("fp32c", False), ("surrogate", True)):
-            t0 = time.perf_counter()
-            f = fork(winds.PREC_FP32C, ws, max_steps=steps + 8)
-            done = 0
-            while done < steps:
-                k = min(per_rev, steps - done)
-                f.step(k)
-                f.sync()
-                done += k
-                log(f"  continued march {name}: {done}/{steps} steps")
-            hist[name] = f.perf()
-            el = time.perf_counter() - t0
-            out[f"continue_{revs}rev_s_{name}"] = round(el, 2)
-            log(f"march {revs} more revs ({steps} steps) {name}: {el:.1f} s")
-            if ws:
-                out["surrogate_stats_continue"] = f.surr_stats()
-            del f
-        a, b = hist["surrogate"], hist["fp32c"]
-        relh = lambda i: float(np.max(np.abs(a[:, i] - b[:, i]) / np.abs(b[:, i])))
+        plan += [("surrogate", winds.PREC_FP32C, 0, steps)]
+    plan += [("fp64", winds.PREC_FP64, None, fp64_revs * per_rev)]
+    for name, prec, mode, nst in plan:
+        t0 = time.perf_counter()
+        f = fork(prec, mode)
+        done = 0
+        while done < nst:
+            k = min(per_rev, nst - done)
+            f.step(k)
+            f.sync()
+            done += k
+            log(f"  continued march {name}: {done}/{nst} steps")
+        hist[name] = f.perf()
+        el = time.perf_counter() - t0
+        out[f"continue_s_{name}"] = round(el, 2)
+        log(f"march {nst} more steps {name}: {el:.1f} s")
+        if mode is not None:
+            out["surrogate_stats_continue"] = f.surr_stats()
+        del f
+
+    def errs(a, b, n):
+        a, b = a[:n], b[:n]
+        rel = lambda i: float(np.max(np.abs(a[:, i] - b[:, i]) / np.abs(b[:, i])))
         meanr = lambda i: float(abs(np.mean(a[:, i]) - np.mean(b[:, i])) / abs(np.mean(b[:, i])))
-        out.update({
-            "surrogate_thrust_max_rel_err_vs_fp32c": relh(1),
-            "surrogate_power_max_rel_err_vs_fp32c": relh(3),
-            "surrogate_thrust_mean_rel_err_vs_fp32c": meanr(1),
-            "surrogate_power_mean_rel_err_vs_fp32c": meanr(3),
-            "thrust_N_mean_fp32c": float(np.mean(b[:, 1])),
-            "power_W_mean_fp32c": float(np.mean(b[:, 3])),
-        })
-        log(f"surrogate vs fp32c over {revs} revs: thrust max {relh(1):.2e} power max {relh(3):.2e}")
-    # (4) cold start: FP64 vs the FP32 cold start above
-    t0 = time.perf_counter()
-    m = March(rot, 1440, march_opts(winds.PREC_FP64), integrator="pcc", max_steps=steps + 8)
-    m.step(steps)
-    m.sync()
-    b = m.perf()
-    del m
-    log(f"march {revs} revs cold start fp64: {time.perf_counter() - t0:.1f} s")
-    a = cold32[:steps]
-    last = slice(-per_rev, None)
-    rel = lambda i: float(np.max(np.abs(a[last, i] - b[last, i]) / np.abs(b[last, i])))
+        return {"thrust_max_rel": rel(1), "power_max_rel": rel(3), "thrust_mean_rel": meanr(1),
+                "power_mean_rel": meanr(3), "steps": int(n)}
+
+    n64 = fp64_revs * per_rev
+    b64 = hist["fp64"]
+    out["fp32c_vs_fp64"] = errs(hist["fp32c"], b64, n64)
+    out["thrust_N_mean_fp64"] = float(np.mean(b64[:, 1]))
+    out["power_W_mean_fp64"] = float(np.mean(b64[:, 3]))
+    out["kj_iterations_mean_fp64"] = float(np.mean(b64[:, 4]))
+    log(f"fp32c vs fp64 over {fp64_revs} revs at full length: {out['fp32c_vs_fp64']}")
+    if surrogate:
+        out["surrogate_vs_fp64"] = errs(hist["surrogate"], b64, n64)
+        out["surrogate_vs_fp32c"] = errs(hist["surrogate"], hist["fp32c"], steps)
+        log(f"surrogate vs fp64 over {fp64_revs} revs: {out['surrogate_vs_fp64']}; "
+            f"vs fp32c over {revs} revs: {out['surrogate_vs_fp32c']}")
     out.update({"revs": revs, "steps": steps,
-                "thrust_N_last_rev_fp64": float(np.mean(b[last, 1])),
-                "power_W_last_rev_fp64": float(np.mean(b[last, 3])),
-                "thrust_max_rel_err_fp32c_vs_fp64_last_rev": rel(1),
-                "power_max_rel_err_fp32c_vs_fp64_last_rev": rel(3),
-                "kj_iterations_mean": float(np.mean(a[:, 4]))})
+                "thrust_N_mean_fp32c": float(np.mean(hist["fp32c"][:, 1])),
+                "power_W_mean_fp32c": float(np.mean(hist["fp32c"][:, 3])),
+                "config5_steps_covered": (f"{dev_steps + steps} of 3600 steps in FP32 "
+                                          f"({dev_steps} cold start + {steps} continued), "
+                                          f"{steps} with the surrogate, {n64} in FP64 from the "
+                                          f"developed 1,440-age state")})
     return out, surr_rep
 
 
@@ -446,6 +453,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-wake batch")
     ap.add_argument("--no-march", action="store_true", help="skip the free-wake time march")
     ap.add_argument("--march-revs", type=int, default=10)
+    ap.add_argument("--fp64-revs", type=int, default=2,
+                    help="revolutions of the FP64 reference march from the developed state")
     ap.add_argument("--develop-revs", type=int, default=20,
                     help="cold-start revolutions before the full-length measurements")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the far-wake network")
@@ -593,7 +602,7 @@ def main():
     if not args.no_march:
         march, surr_rep = march_bench(args.march_revs, args.develop_revs, not args.no_surrogate,
                                       args.surrogate_samples, args.surrogate_steps,
-                                      args.check_every)
+                                      args.check_every, args.fp64_revs)
 
     achieved = inter * FLOP_VATISTAS2 / (ms * 1e-3) / 1e12
     line = {

@@ -5,22 +5,27 @@
  * kernels on wake state that never leaves the GPU:
  *
  *   setup             core/initials.m:44-152 (two ages, bound circulation
- *                     0.5*|U|*chord*cl(geometric aoa), r0, vcore, induced
- *                     velocity, forward Euler)
+ *                     0.5*|U(1)|*chord*cl_init, r0, vcore, induced velocity,
+ *                     forward Euler)
  *   per step p        WInDS.m:127-183: Kelvin circulation update
  *                     (WInDS.m:131-140), vcore + filamentmod (core/vcore.m:
  *                     30-51, core/filamentmod.m:27-41), shed + trailing
  *                     induced velocity at every node (the BiotSavartMex calls
  *                     of WInDS.m:150-154), integrator (numerical/pcc.m:31-75,
  *                     fe.m:30-31, ab4.m:34-42), Kutta-Joukowski fixed point
- *                     in FP64 (core/KuttaJoukowski.m:42-49, 173-298) and rotor
- *                     thrust/torque/power (postproc/perform.m:42-62).
+ *                     in FP64 with each station's own airfoil table
+ *                     (core/KuttaJoukowski.m:42-49, 173-298) and rotor
+ *                     thrust/torque/power (postproc/perform.m:16-62).
  *
- * The rotor is rigid (constant Omega, no platform motion): blade b at
- * azimuth 2 pi b/nb + Omega t, quarter chord r e_r, trailing edge
- * r e_r + 0.75 c e_c, blade frame bxn = e_c, bzn = e_r, byn = bzn x bxn with
- * e_c = -cos(theta) e_t + sin(theta) e_x, theta = twist + pitch.  The wake
- * keeps at most max_ages ages; older ages are dropped.
+ * The motion of the rotor comes in as tables per time index -- what the
+ * reference computes before its time loop (core/kinematics.m, core/
+ * velocity.m; WInDS.m:112-116) -- so any platform, nacelle, shaft and blade
+ * motion the caller states drives the march: the first two rows of wake
+ * nodes (pos.quarter, pos.trail), the lifting-line points (pos.bound), the
+ * blade frames there (pos.nodes.bxn/byn/bzn) and the kinematic inflow in
+ * that frame (vel.blade), with fastout's time, azimuth, tip speed ratio,
+ * rotor speed and blade pitch and the free stream wind.infty.  The wake keeps
+ * at most max_ages ages; older ages are dropped.
  */
 #ifndef WINDS_MARCH_H
 #define WINDS_MARCH_H
@@ -40,31 +45,54 @@ extern "C" {
 
 typedef struct winds_march winds_march;
 
+/* Arrays are column-major (MATLAB's layout); [ns,3,nt,nb] means element
+ * (m, c, p, b) at m + ns*(c + 3*(p + nt*b)).  ns = nst - 1 stations. */
 typedef struct winds_march_cfg {
-    int32_t nb;            /* blades                                           */
-    int32_t nst;           /* trailing nodes per blade (ns = nst-1 stations)   */
-    int32_t max_ages;      /* wake truncation (>= 3)                           */
-    int32_t n_alpha;       /* polar table length, 2..1024                      */
-    const double *r_trail;      /* [nst] node radii (blade.RTrail)             */
-    const double *chord_trail;  /* [nst] (blade.ChordTrail)                    */
-    const double *twist_trail;  /* [nst] degrees (blade.AeroTwstTrail)         */
-    const double *alpha;        /* [n_alpha] degrees, increasing               */
-    const double *cl;           /* [n_alpha]                                   */
-    const double *cd;           /* [n_alpha]                                   */
-    double R;              /* tip radius (initials.m:134)                      */
-    double tsr;            /* tip speed ratio (initials.m:134)                 */
-    double omega;          /* rad/s                                            */
-    double dt;             /* time step, s                                     */
-    double pitch_deg;      /* blade pitch                                      */
-    double uinf[3];        /* free stream, m/s                                 */
-    double relax;          /* KJ relaxation (WInDS.m:53)                       */
-    double tol;            /* KJ tolerance (WInDS.m:41)                        */
-    int32_t maxiter;       /* KJ iterations (WInDS.m:47)                       */
-    int32_t integrator;    /* WINDS_INTEG_*                                    */
-    int32_t max_steps;     /* capacity of the performance history             */
-    int32_t reserved;
-    winds_bs_opts wake;    /* evaluation of the wake on its own nodes          */
-    winds_bs_opts ll;      /* evaluation at the lifting line (KJ)              */
+    int32_t nb;            /* blades (turbine.NumBl)                              */
+    int32_t nst;           /* trailing nodes per blade (length(blade.RTrail))     */
+    int32_t max_ages;      /* wake truncation (>= 3)                              */
+    int32_t n_times;       /* time indices the tables hold (length(fastout.Time)) */
+    /* blade (NRELrotor.m:34-55 names), [ns] each */
+    const double *r_node;      /* blade.RNodes                                    */
+    const double *chord_node;  /* blade.Chord                                     */
+    const double *twist_node;  /* blade.AeroTwst, degrees                         */
+    const double *dr_node;     /* blade.DRNodes                                   */
+    const int32_t *foil;       /* blade.NFoil, 1-based airfoil table per station  */
+    const double *cl_init;     /* initial lift coefficient (initials.m:111-119:   */
+                               /*   BEM.m, or the elliptic wing's closed form)    */
+    double tip_radius;         /* blade.TipRad (initials.m:134)                   */
+    /* airfoil tables (airfoils.profiles): n_foils tables, concatenated */
+    int32_t n_foils;           /* 1..64                                           */
+    int32_t n_alpha;           /* total entries, 2..1024                          */
+    const int32_t *foil_len;   /* [n_foils] entries of each table (>= 2)          */
+    const double *alpha;       /* [n_alpha] degrees, increasing within a table    */
+    const double *cl;          /* [n_alpha]                                       */
+    const double *cd;          /* [n_alpha]                                       */
+    /* per time index */
+    const double *time;        /* [nt] fastout.Time, s                            */
+    const double *wind;        /* [nt,3] wind.infty, m/s                          */
+    const double *tsr;         /* [nt] fastout.TipSpdRat                          */
+    const double *rot_speed;   /* [nt] fastout.RotSpeed, rpm                      */
+    const double *azimuth;     /* [nt] fastout.Azimuth, degrees                   */
+    const double *pitch;       /* [nt,nb] fastout.BldPitch1..nb, degrees          */
+    const double *hub;         /* [nt,3] pos.hub                                  */
+    const double *quarter;     /* [nst,3,nt,nb] pos.quarter (wake age 0)          */
+    const double *trail;       /* [nst,3,nt,nb] pos.trail (wake age 1)            */
+    const double *bound;       /* [ns,3,nt,nb] pos.bound (lifting line)           */
+    const double *bxn;         /* [ns,3,nt,nb] pos.nodes.bxn                      */
+    const double *byn;         /* [ns,3,nt,nb] pos.nodes.byn                      */
+    const double *bzn;         /* [ns,3,nt,nb] pos.nodes.bzn                      */
+    const double *vblade;      /* [ns,3,nt,nb] vel.blade                          */
+    const double *ll_extra;    /* [ns,3,nt,nb] induced velocity of other sources  */
+                               /*   at the lifting line (vel.uind_tower,          */
+                               /*   KuttaJoukowski.m:232-238), or NULL            */
+    double dt;             /* time step 1/user.t(3), s (fe.m:30)                  */
+    double relax;          /* KJ relaxation (WInDS.m:53)                          */
+    double tol;            /* KJ tolerance (WInDS.m:41)                           */
+    int32_t maxiter;       /* KJ iterations (WInDS.m:47)                          */
+    int32_t integrator;    /* WINDS_INTEG_*                                       */
+    winds_bs_opts wake;    /* evaluation of the wake on its own nodes             */
+    winds_bs_opts ll;      /* evaluation at the lifting line (KJ)                 */
 } winds_march_cfg;
 
 /* Build the state and run the reference's setup (time index 1).  Enqueued on
@@ -72,7 +100,9 @@ typedef struct winds_march_cfg {
 int winds_march_create(const winds_march_cfg *cfg, winds_march **out);
 void winds_march_destroy(winds_march *m);
 
-/* Advance nsteps time steps (WInDS.m loop body), asynchronously. */
+/* Advance nsteps time steps (WInDS.m loop body), asynchronously.  Stepping
+ * time index p needs the tables of p+1 (the integrator places the new first
+ * two rows of nodes there); beyond that the call fails. */
 int winds_march_step(winds_march *m, int32_t nsteps);
 
 /* Wait for queued work; report a diverging Kutta-Joukowski solve
@@ -87,7 +117,8 @@ int winds_march_info(const winds_march *m, int64_t out[6]);
  * [ns,1,ages,nb]; trailing circulation [nst,1,ages-1,nb]; each may be NULL. */
 int winds_march_get_state(winds_march *m, double *nodes, double *gshed, double *gtrail);
 
-/* Start from a prepared wake: nodes [nst,3,ages,nb] of time index p (>= 4)
+/* Start from a prepared wake: nodes [nst,3,ages,nb] of time index p (>= 4,
+ * p < n_times)
  * and the circulation of time p-1, shed [ns,1,ages-1,nb] (bound = age 1)
  * and trailing [nst,1,ages-2,nb], as the previous step would leave them. */
 int winds_march_set_state(winds_march *m, int32_t p, int32_t ages, const double *nodes,

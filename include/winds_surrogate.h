@@ -46,6 +46,7 @@ typedef struct winds_surr_geom {
     double R;                /* tip radius                                  */
     double U;                /* free-stream speed                           */
     double psi;              /* azimuth of blade 0 (rad), about +x          */
+    double hub[3];           /* rotor centre the positions are taken from   */
 } winds_surr_geom;
 
 /* feat [np][WINDS_SURR_NIN] for the points P (SoA [3][np]); desc is a device

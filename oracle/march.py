@@ -1,30 +1,34 @@
-"""CPU restatement of the WInDS free-wake time march for a rigid rotor.
+"""CPU restatement of the WInDS free-wake time march.
 
 TEST INFRASTRUCTURE ONLY (the checker of the device march); the engine never
 imports it.  Arrays keep MATLAB's shapes and every step follows the reference
 files literally:
 
-  setup            core/initials.m:44-152 (two ages at t1, bound circulation
-                   0.5*|U|*chord*cl, r0 from T0 = 2 pi R / (12 TSR |U|), vcore,
-                   induced velocity, forward Euler)
+  setup            core/initials.m:44-152 (two ages at time index 1, bound
+                   circulation 0.5*|U(1)|*chord*cl_init, r0(t) from
+                   T0 = 2 pi R / (12 TSR(t) |U(t)|), vcore, induced velocity,
+                   forward Euler)
   time loop        WInDS.m:127-183 (Kelvin update, vcore, shed + trail
                    induced velocity at every node, integrator, Kutta-Joukowski)
   Kelvin           WInDS.m:131-140 / pcc.m:43-49 / KuttaJoukowski.m:285-288
-  vcore            core/vcore.m:30-51 (global time, not filament age)
+  vcore            core/vcore.m:30-51 (global time fastout.Time, not age)
   filamentmod      core/filamentmod.m:27-41 (sqrt of the already-square-rooted
                    lengths, strain between equal age indices of two times)
   fe / pcc / ab4   numerical/fe.m:30-31, pcc.m:31-75, ab4.m:34-42
-  Kutta-Joukowski  core/KuttaJoukowski.m:42-49 (fixed point) and kj() :173-298
-  perform          postproc/perform.m:42-62
+  Kutta-Joukowski  core/KuttaJoukowski.m:42-49 (fixed point) and kj() :173-298,
+                   each station interpolating its own airfoil's table
+                   (blade.NFoil, :262-269), other sources at the lifting line
+                   added to the induced velocity (:232-238)
+  perform          postproc/perform.m:16-62 (velX = -DCMRot(vel.blade,
+                   [-90 AeroTwst BldPitch], 'zzz'))
 
-Rigid-rotor kinematics (core/kinematics.m, core/velocity.m) are restated
-analytically: blade b at azimuth 2 pi b/nb + Omega t, quarter chord r e_r,
-trailing edge r e_r + 0.75 c e_c, bxn = e_c, bzn = e_r, byn = bzn x bxn, and
-the blade-station inflow U - Omega r e_t (the reference differentiates the
-positions numerically, ctdiff.m).  When the wake reaches max_ages the oldest
-age is dropped; filamentmod then compares equal age indices 2..end-1 of both
-times (the reference never truncates).  Biot-Savart sums come from the C
-oracle (oracle/bs_oracle.c) in the MEX or BiotSavart.m semantics.
+The rotor's motion comes from the same kinematic tables the device march
+takes (winds.cases.tables: pos.quarter/trail/bound, pos.nodes.bxn/byn/bzn,
+vel.blade, fastout, wind.infty per time index).  When the wake reaches
+max_ages the oldest age is dropped; filamentmod then compares equal age
+indices 2..end-1 of both times (the reference never truncates).  Biot-Savart
+sums come from the C oracle (oracle/bs_oracle.c) in the MEX or BiotSavart.m
+semantics.
 """
 from __future__ import annotations
 
@@ -42,53 +46,52 @@ RHO = 1.23           # :28
 
 
 class OracleMarch:
-    def __init__(self, rotor, max_ages: int, d=1e-4, co: float = math.inf,
+    def __init__(self, tables, max_ages: int, d=1e-4, co: float = math.inf,
                  semantics: str = "mex", integ: str = "pcc", relax: float = 0.25,
-                 tol: float = 1e-8, maxiter: int = 30, threads: int | None = None):
-        self.r = rotor
+                 tol: float = 1e-8, maxiter: int = 30, threads: int | None = None,
+                 setup: bool = True):
+        self.tb = tables
         self.max_ages = max_ages
         self.d, self.co, self.sem, self.integ = d, co, semantics, integ
         self.relax, self.tol, self.maxiter = relax, tol, maxiter
         self.threads = threads
-        self.nb, self.nst, self.ns = rotor.nb, rotor.nst, rotor.ns
-        self.dt, self.om = rotor.dt, rotor.omega
-        self.U = rotor.uinf
+        self.nb, self.nst, self.ns = tables.nb, tables.nst, tables.ns
+        self.dt = tables.dt
         self.history = []          # (p, T, Q, P, kj iterations)
-        self._setup()
+        # offsets of each airfoil table in the concatenated arrays
+        self.foil_off = np.concatenate([[0], np.cumsum(tables.foil_len)[:-1]]).astype(int)
+        if setup:
+            self._setup()
 
-    # ------------------------------------------------------------------ kinematics
-    def _frames(self, t):
-        psi = 2 * math.pi * np.arange(self.nb) / self.nb + self.om * t
-        er = np.stack([np.zeros(self.nb), np.cos(psi), np.sin(psi)], axis=1)    # [nb,3]
-        et = np.stack([np.zeros(self.nb), -np.sin(psi), np.cos(psi)], axis=1)
-        ex = np.array([1.0, 0.0, 0.0])
-        return er, et, ex
+    # ------------------------------------------------------------------ tables
+    def wind(self, p):
+        return np.asarray(self.tb.wind)[p - 1]
 
-    def _ec(self, et, ex, theta_deg):
-        th = np.radians(theta_deg + self.r.pitch_deg)                              # [n]
-        return (-np.cos(th)[:, None, None] * et[None, :, :]
-                + np.sin(th)[:, None, None] * ex[None, None, :])                   # [n,nb,3]
+    def r0(self, p):
+        """initials.m:134-135 at time index p."""
+        T0 = 2 * math.pi * self.tb.tip_radius / (12 * self.tb.tsr[p - 1] * np.linalg.norm(self.wind(p)))
+        return math.sqrt(4 * ALPHA_RL * NU * DELTA * T0)
 
-    def quarter_trail(self, t):
-        """pos.quarter and pos.trail, [nst,3,nb] each (kinematics.m:49-50)."""
-        er, et, ex = self._frames(t)
-        r = self.r.r_trail
-        q = r[:, None, None] * er[None, :, :]                                     # [nst,nb,3]
-        tr = q + 0.75 * self.r.chord_trail[:, None, None] * self._ec(et, ex, self.r.twist_trail)
-        return np.transpose(q, (0, 2, 1)), np.transpose(tr, (0, 2, 1))
+    def quarter_trail(self, p):
+        """pos.quarter and pos.trail at time index p, [nst,3,nb] each."""
+        return self.tb.quarter[:, :, p - 1, :], self.tb.trail[:, :, p - 1, :]
 
-    def lifting_line(self, t):
-        """pos.bound [ns,3,nb], frames bxn/byn/bzn [ns,3,nb], vel.blade [ns,3,nb]."""
-        er, et, ex = self._frames(t)
-        rn = self.r.r_node
-        bound = rn[:, None, None] * er[None, :, :]
-        bx = self._ec(et, ex, self.r.twist_node)                                  # [ns,nb,3]
-        bz = np.broadcast_to(er[None, :, :], bx.shape)
-        by = np.cross(bz, bx)
-        vb = self.U[None, None, :] - (self.om * rn)[:, None, None] * et[None, :, :]
-        vblade = np.stack([np.sum(bx * vb, -1), np.sum(by * vb, -1), np.sum(bz * vb, -1)], -1)
-        tp = lambda a: np.transpose(a, (0, 2, 1))
-        return tp(bound), tp(bx), tp(by), tp(bz), tp(vblade), tp(vb)
+    def lifting_line(self, p):
+        """pos.bound, bxn, byn, bzn, vel.blade at time index p, [ns,3,nb] each."""
+        t = self.tb
+        return (t.bound[:, :, p - 1, :], t.bxn[:, :, p - 1, :], t.byn[:, :, p - 1, :],
+                t.bzn[:, :, p - 1, :], t.vblade[:, :, p - 1, :])
+
+    def polar(self, table, aoa):
+        """cl or cd of every station from its airfoil (KuttaJoukowski.m:262-269);
+        aoa [ns, nb]."""
+        t = self.tb
+        out = np.empty_like(aoa)
+        for m in range(self.ns):
+            k = int(t.foil[m]) - 1
+            o, n = self.foil_off[k], int(t.foil_len[k])
+            out[m] = np.interp(aoa[m], t.alpha[o:o + n], table[o:o + n], left=np.nan, right=np.nan)
+        return out
 
     # ------------------------------------------------------------------ pieces
     def _bs(self, F1, F2, P, g, rc):
@@ -121,11 +124,12 @@ class OracleMarch:
 
     def vcore(self, p, dom, gs, gt, len_prev):
         """vcore.m:30-51 with filamentmod.m:27-41; returns rc_eff and lengths."""
-        T = (p - 1) * self.dt
+        T = float(self.tb.time[p - 1])
+        r0 = self.r0(p)
         res = np.abs(gs / NU)
         ret = np.abs(gt / NU)
-        rcs = self.r0 ** 2 + 4 * ALPHA_RL * NU * (1 + A1 * res) * T
-        rct = self.r0 ** 2 + 4 * ALPHA_RL * NU * (1 + A1 * ret) * T
+        rcs = r0 ** 2 + 4 * ALPHA_RL * NU * (1 + A1 * res) * T
+        rct = r0 ** 2 + 4 * ALPHA_RL * NU * (1 + A1 * ret) * T
         ls = O.filament_length(dom[:-1], dom[1:])
         lt = O.filament_length(dom[:, :, 1:], dom[:, :, :-1])
         rce_s, rce_t = rcs.copy(), rct.copy()
@@ -143,9 +147,11 @@ class OracleMarch:
             rce_s[:, :, ish, :] = rcs[:, :, ish, :] * (1.0 / (1 + ss))
         return rce_s, rce_t, (ls, lt)
 
-    def _advect(self, dom, vel, t_next, vel_next=None, hist=None):
+    def _advect(self, dom, vel, p_next, vel_next=None, hist=None):
+        """Nodes of time index p_next: ages 0 and 1 from the tables, older ages
+        moved by fe.m, pcc.m's corrector (vel_next) or ab4.m (hist)."""
         A_new = min(dom.shape[2] + 1, self.max_ages)
-        q, tr = self.quarter_trail(t_next)
+        q, tr = self.quarter_trail(p_next)
         out = np.empty((self.nst, 3, A_new, self.nb))
         out[:, :, 0, :] = q
         out[:, :, 1, :] = tr
@@ -166,57 +172,51 @@ class OracleMarch:
             out[:, :, 2:, :] = src + self.dt * vel[:, :, 1:1 + n, :]
         return np.asfortranarray(out)
 
-    def interp(self, table, aoa):
-        return np.interp(aoa, self.r.alpha, table, left=np.nan, right=np.nan)
-
     # ------------------------------------------------------------------ setup
     def _setup(self):
-        r = self.r
-        self.r0 = math.sqrt(4 * ALPHA_RL * NU * DELTA * (2 * math.pi * r.R / (12 * r.tsr * np.linalg.norm(self.U))))
-        q, tr = self.quarter_trail(0.0)
+        q, tr = self.quarter_trail(1)
         dom = np.empty((self.nst, 3, 2, self.nb))
         dom[:, :, 0, :], dom[:, :, 1, :] = q, tr
         self.dom = np.asfortranarray(dom)
-        _, bx, by, bz, vblade, _ = self.lifting_line(0.0)
-        aoa = np.degrees(np.arctan2(-vblade[:, 1, :], vblade[:, 0, :]))           # [ns,nb]
-        cl0 = self.interp(r.cl, aoa)
-        self.gbound = 0.5 * np.linalg.norm(self.U) * r.chord_node[:, None] * cl0   # initials.m:124
+        U1 = np.linalg.norm(self.wind(1))
+        cl0 = np.asarray(self.tb.cl_init)
+        self.gbound = np.repeat((0.5 * U1 * np.asarray(self.tb.chord_node) * cl0)[:, None],
+                                self.nb, axis=1)                                 # initials.m:124
         self.gs, self.gt = self.kelvin(self.gbound, np.zeros((self.nst, 1, 0, self.nb)), 2)
         rce_s, rce_t, lens = self.vcore(1, self.dom, self.gs, self.gt, None)
         u = self.induced(self.dom, self.gs, self.gt, rce_s, rce_t, self.dom)
-        vel = self.U[None, :, None, None] + u
+        vel = self.wind(1)[None, :, None, None] + u
         self.lens = lens
         self.vel_hist = [vel]
-        self.dom = self._advect(self.dom, vel, self.dt)
+        self.dom = self._advect(self.dom, vel, 2)
         self.p = 2
 
     # ------------------------------------------------------------------ one step
     def step(self):
         p = self.p
-        t = (p - 1) * self.dt
         A = self.dom.shape[2]
         self.gs, self.gt = self.kelvin(self.gbound, self.gt, A)
         rce_s, rce_t, lens = self.vcore(p, self.dom, self.gs, self.gt, self.lens)
         u = self.induced(self.dom, self.gs, self.gt, rce_s, rce_t, self.dom)
-        vel = self.U[None, :, None, None] + u
+        vel = self.wind(p)[None, :, None, None] + u
         # integrate to p+1
         if self.integ == "pcc":
-            pred = self._advect(self.dom, vel, t + self.dt)
+            pred = self._advect(self.dom, vel, p + 1)
             if p > 2:
                 gs1, gt1 = self.kelvin(self.gbound, self.gt, pred.shape[2])
                 rs1, rt1, _ = self.vcore(p + 1, pred, gs1, gt1, lens)
                 u1 = self.induced(pred, gs1, gt1, rs1, rt1, pred)
-                vel1 = self.U[None, :, None, None] + u1
-                pred = self._advect(self.dom, vel, t + self.dt, vel_next=vel1)
+                vel1 = self.wind(p + 1)[None, :, None, None] + u1
+                pred = self._advect(self.dom, vel, p + 1, vel_next=vel1)
             dom_next = pred
         elif self.integ == "ab4" and p >= 4:
             h = self.vel_hist
-            dom_next = self._advect(self.dom, vel, t + self.dt, hist=(vel, h[-1], h[-2], h[-3]))
+            dom_next = self._advect(self.dom, vel, p + 1, hist=(vel, h[-1], h[-2], h[-3]))
         else:
-            dom_next = self._advect(self.dom, vel, t + self.dt)
+            dom_next = self._advect(self.dom, vel, p + 1)
         # Kutta-Joukowski fixed point at time p
-        iters, cl, cd, aoa = self.kj(t, rce_s, rce_t)
-        T, Q, Pw = self.perform(cl, cd, aoa, t)
+        iters, cl, cd, aoa = self.kj(p, rce_s, rce_t)
+        T, Q, Pw = self.perform(cl, cd, aoa, p)
         self.history.append((p, T, Q, Pw, iters))
         self.last = dict(cl=cl, cd=cd, aoa=aoa, vel=vel, rce_s=rce_s, rce_t=rce_t)
         self.lens = lens
@@ -224,20 +224,24 @@ class OracleMarch:
         self.dom = dom_next
         self.p = p + 1
 
-    def kj(self, t, rce_s, rce_t):
-        bound, bx, by, bz, vblade, _ = self.lifting_line(t)
+    def kj(self, p, rce_s, rce_t):
+        bound, bx, by, bz, vblade = self.lifting_line(p)
         P = np.asfortranarray(bound.reshape(self.ns, 3, 1, self.nb))
+        extra = None
+        if getattr(self.tb, "ll_extra", None) is not None:
+            extra = self.tb.ll_extra[:, :, p - 1, :]
         Vinf = np.linalg.norm(vblade, axis=1)                                      # [ns,nb]
-        chord = self.r.chord_node[:, None]
+        chord = np.asarray(self.tb.chord_node)[:, None]
         dg_max, j = 1.0, 0
-        A = self.dom.shape[2]
         while dg_max > self.tol and j < self.maxiter:
             ub = self.induced(self.dom, self.gs, self.gt, rce_s, rce_t, P)[:, :, 0, :]  # [ns,3,nb]
+            if extra is not None:
+                ub = ub + extra
             rot = np.stack([np.sum(bx * ub, 1), np.sum(by * ub, 1), np.sum(bz * ub, 1)], 1)
             tot = vblade + rot
             aoa = np.arctan2(-tot[:, 1, :], tot[:, 0, :]) * (180 / np.pi)
-            cl = self.interp(self.r.cl, aoa)
-            cd = self.interp(self.r.cd, aoa)
+            cl = self.polar(self.tb.cl, aoa)
+            cd = self.polar(self.tb.cd, aoa)
             if np.any(np.isnan(cl)):
                 raise FloatingPointError("Diverging soln!!!")
             gnew = 0.5 * Vinf * chord * cl
@@ -255,16 +259,28 @@ class OracleMarch:
             j += 1
         return j, cl, cd, aoa
 
-    def perform(self, cl, cd, aoa, t):
-        """perform.m:42-62 with velX = (axial inflow, in-plane speed) of the
-        rigid rotor, phi = aoa - (pitch + twist)."""
-        _, et_, ex = self._frames(t)
-        rn = self.r.r_node
-        vb = self.U[None, None, :] - (self.om * rn)[:, None, None] * et_[None, :, :]   # [ns,nb,3]
-        v1 = vb[..., 0]
-        v2 = -np.sum(vb * et_[None], -1)
+    def velX(self, p):
+        """perform.m:19-24: vel.blade rotated by -90 deg, twist and pitch about
+        z (DCMRot 'zzz', which composes to one rotation) and negated; [ns,3,nb]."""
+        _, _, _, _, vb = self.lifting_line(p)
+        tw = np.asarray(self.tb.twist_node)[:, None]
+        pitch = np.asarray(self.tb.pitch).reshape(self.tb.n_times, -1)[p - 1][None, :]
+        th = np.radians(-90.0 + tw + pitch)                                          # [ns,nb]
+        c, s = np.cos(th), np.sin(th)
+        out = np.empty_like(vb)
+        out[:, 0, :] = -(c * vb[:, 0, :] - s * vb[:, 1, :])
+        out[:, 1, :] = -(s * vb[:, 0, :] + c * vb[:, 1, :])
+        out[:, 2, :] = -vb[:, 2, :]
+        return out
+
+    def perform(self, cl, cd, aoa, p):
+        """perform.m:16-62 for time index p."""
+        vX = self.velX(p)
+        v1, v2 = vX[:, 0, :], vX[:, 1, :]
         tsr = v2 / v1
-        phi = aoa - (self.r.pitch_deg + self.r.twist_node[:, None])
+        tw = np.asarray(self.tb.twist_node)[:, None]
+        pitch = np.asarray(self.tb.pitch).reshape(self.tb.n_times, -1)[p - 1][None, :]
+        phi = aoa - (pitch + tw)
         tphi = np.tan(np.radians(phi))
         aap = tsr / tphi
         a = aap * (v1 - tphi * v2) / (aap * v1 + tphi * v2)
@@ -273,9 +289,10 @@ class OracleMarch:
         dCZ = cl * cph + cd * sph
         dCX = cl * sph - cd * cph
         W2 = (v1 * (1 - a)) ** 2 + (v2 * (1 + ap)) ** 2
-        c = self.r.chord_node[:, None]
-        dr = self.r.dr_node[:, None]
+        c = np.asarray(self.tb.chord_node)[:, None]
+        dr = np.asarray(self.tb.dr_node)[:, None]
+        rn = np.asarray(self.tb.r_node)[:, None]
         dT = 0.5 * RHO * c * dCZ * W2 * dr
-        dQ = 0.5 * RHO * c * dCX * W2 * rn[:, None] * dr
-        dP = self.om * dQ          # (pi/30) * RotSpeed[rpm] = Omega
+        dQ = 0.5 * RHO * c * dCX * W2 * rn * dr
+        dP = (math.pi / 30) * float(self.tb.rot_speed[p - 1]) * dQ
         return float(dT.sum()), float(dQ.sum()), float(dP.sum())

@@ -3,38 +3,48 @@
 The FP64 march must follow the oracle march (oracle/march.py, which restates
 WInDS.m, initials.m, pcc.m/fe.m/ab4.m, vcore.m, filamentmod.m,
 KuttaJoukowski.m and perform.m) to 1e-9 relative over every step, including
-steps after the wake reaches its maximum length.  Sums differ only in order
-(the device splits the lifting-line sum into the filaments fixed during the
-Kutta-Joukowski iteration and those it changes)."""
+steps after the wake reaches its maximum length.  Both read the same
+kinematic tables (winds.cases.tables).  Sums differ only in order (the device
+splits the lifting-line sum into the filaments fixed during the
+Kutta-Joukowski iteration and those it changes).  Cases: a small synthetic
+rotor in every integrator and core, the NREL 5-MW rotor (eight airfoil
+tables, shaft tilt, precone, tower height and overhang), the same rotor on a
+surging and pitching platform with yaw, and the 3 x 31-node, 5-degree
+geometry of the full-length benchmark march."""
 import math
+from functools import lru_cache
 
 import numpy as np
 import pytest
 
 from conftest import flush_print
 import winds
-from winds import rotor as rmod
+from winds import cases
 
 pytestmark = pytest.mark.gpu
 
 
-def small_rotor(**kw):
-    return rmod.synthetic_rotor(nb=2, nst=7, R=20.0, hub=1.0, tsr=6.0, U=10.0, dpsi_deg=10.0, **kw)
+@lru_cache(maxsize=None)
+def small_tables(n_times=40):
+    return cases.tables(cases.synthetic_rotor(nb=2, nst=7, R=20.0, hub=1.0, tsr=6.0, U=10.0,
+                                              dpsi_deg=10.0, n_times=n_times))
 
 
-def run_pair(rot, max_ages, steps, sem, d, integ):
+def wake_opts(sem, d, prec=winds.PREC_FP64):
+    if sem == "mex":
+        return winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, precision=prec, d=d)
+    if isinstance(d, str):
+        return winds.opts(core=winds.CORE_VATISTAS, vat_n=int(d[4:]), semantics=winds.SEM_MATLAB,
+                          precision=prec)
+    return winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MATLAB, precision=prec, d=d,
+                      co=1000.0)
+
+
+def run_pair(tb, max_ages, steps, sem, d, integ):
     from oracle.march import OracleMarch
     from winds.march import March
-    if sem == "mex":
-        o = winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, precision=winds.PREC_FP64, d=d)
-    elif isinstance(d, str):
-        o = winds.opts(core=winds.CORE_VATISTAS, vat_n=int(d[4:]), semantics=winds.SEM_MATLAB,
-                       precision=winds.PREC_FP64)
-    else:
-        o = winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MATLAB, precision=winds.PREC_FP64,
-                       d=d, co=1000.0)
-    m = March(rot, max_ages, o, integrator=integ)
-    om = OracleMarch(rot, max_ages, d=d, co=1000.0 if sem != "mex" else math.inf,
+    m = March(tb, max_ages, wake_opts(sem, d), integrator=integ)
+    om = OracleMarch(tb, max_ages, d=d, co=1000.0 if sem != "mex" else math.inf,
                      semantics=sem, integ=integ)
     for _ in range(steps):
         m.step(1)
@@ -43,12 +53,7 @@ def run_pair(rot, max_ages, steps, sem, d, integ):
     return m, om
 
 
-@pytest.mark.parametrize("sem,d,integ", [("mex", 1e-4, "pcc"), ("matlab", "visc2", "pcc"),
-                                         ("mex", 1e-4, "ab4"), ("matlab", 1e-4, "fe")])
-def test_march_fp64_matches_oracle(need_gpu, sem, d, integ):
-    rot = small_rotor()
-    steps, max_ages = 14, 10        # the wake hits max_ages after 8 steps
-    m, om = run_pair(rot, max_ages, steps, sem, d, integ)
+def compare(m, om, label):
     nodes, _, _ = m.state()
     assert nodes.shape == om.dom.shape
     scale = np.max(np.abs(om.dom))
@@ -61,27 +66,76 @@ def test_march_fp64_matches_oracle(need_gpu, sem, d, integ):
     dT = np.max(np.abs(perf[:, 1:4] - ref[:, 1:4]) / np.abs(ref[:, 1:4]))
     cl, cd, aoa = m.kj()
     dcl = np.max(np.abs(cl - om.last["cl"])) / np.max(np.abs(om.last["cl"]))
+    dcd = np.max(np.abs(cd - om.last["cd"])) / np.max(np.abs(om.last["cd"]))
     gs, gt = m.state_circulation(om.gs.shape[2])
     dg = max(np.max(np.abs(gs - om.gs)), np.max(np.abs(gt - om.gt))) / np.max(np.abs(om.gs))
-    flush_print(f"[march {sem} {d} {integ}] nodes {dn:.2e} perf {dT:.2e} cl {dcl:.2e} gamma {dg:.2e}"
-                f" T={ref[-1, 1]:.4g}")
-    assert dn <= 1e-9 and dT <= 1e-9 and dcl <= 1e-9 and dg <= 1e-9
+    flush_print(f"[march {label}] nodes {dn:.2e} perf {dT:.2e} cl {dcl:.2e} cd {dcd:.2e} "
+                f"gamma {dg:.2e} T={ref[-1, 1]:.4g} P={ref[-1, 3]:.4g}")
+    assert dn <= 1e-9 and dT <= 1e-9 and dcl <= 1e-9 and dcd <= 1e-9 and dg <= 1e-9
+
+
+@pytest.mark.parametrize("sem,d,integ", [("mex", 1e-4, "pcc"), ("matlab", "visc2", "pcc"),
+                                         ("mex", 1e-4, "ab4"), ("matlab", 1e-4, "fe")])
+def test_march_fp64_matches_oracle(need_gpu, sem, d, integ):
+    steps, max_ages = 14, 10        # the wake hits max_ages after 8 steps
+    m, om = run_pair(small_tables(), max_ages, steps, sem, d, integ)
+    compare(m, om, f"{sem} {d} {integ}")
+
+
+def test_nrel_5mw_rotor_matches_oracle(need_gpu):
+    """modeldata/NRELrotor.m at WInDS.m's defaults (TSR 7, 11.4 m/s, 10 Hz):
+    18 stations on eight airfoils (two cylinders, five DU and the NACA64
+    tables), shaft tilt 5 deg, precone -2.5 deg, hub 90 m up and 5 m over."""
+    tb = cases.tables(cases.nrel_rotor(t=(0.0, 2.0, 10.0)))
+    assert sorted(set(tb.foil.tolist())) == list(range(1, 9))
+    m, om = run_pair(tb, 12, 16, "mex", 1e-4, "pcc")
+    compare(m, om, "NREL 5-MW")
+
+
+def test_nrel_5mw_on_a_moving_platform_matches_oracle(need_gpu):
+    """The same rotor yawed 10 deg on a platform that surges 2 m and pitches
+    3 deg at 0.2-0.3 Hz (NRELrotor.m:92-123 modes), blade pitch 2 deg,
+    Vatistas core: the kinematic tables drive the march."""
+    modes = (("surge", 2.0, 0.2, 0.0, 0.5, 0.3, 0.4, 0.0, 0.0),
+             ("pitch", 3.0, 0.25, 0.2, 0.0, 0.0, 0.0, 1.0, 0.0))
+    tb = cases.tables(cases.nrel_rotor(yaw=10.0, pitch=2.0, modes=modes, t=(0.0, 2.0, 10.0)))
+    assert np.ptp(tb.hub[:, 0]) > 1.0                      # the hub moves with the platform
+    m, om = run_pair(tb, 12, 16, "matlab", "visc2", "pcc")
+    compare(m, om, "NREL 5-MW moving platform")
+
+
+def test_full_length_geometry_matches_oracle(need_gpu):
+    """3 blades x 31 nodes at 5-degree steps with PCC, the geometry of the
+    1,440-age benchmark march, over 34 steps with a 30-age wake (the wake is
+    truncated for the last 6)."""
+    tb = cases.tables(cases.synthetic_rotor(nb=3, nst=31, n_times=40))
+    m, om = run_pair(tb, 30, 34, "mex", 1e-4, "pcc")
+    compare(m, om, "3 x 31, 5 deg")
 
 
 def test_zero_lift_wake_convects_with_the_free_stream(need_gpu):
+    """No lift: no circulation and no induction, so a node of age a >= 1
+    sits where the trailing edge was a-1 steps ago, moved by the free
+    stream (fe.m/pcc.m with vel.domain = wind.infty)."""
     from winds.march import March
-    rot = small_rotor()
-    rot.cl = np.zeros_like(rot.cl)          # no lift: no circulation, no induction
-    m = March(rot, 9, winds.opts(d=1e-4))
+    tb = cases.tables(cases.synthetic_rotor(nb=2, nst=7, R=20.0, hub=1.0, tsr=6.0, U=10.0,
+                                            dpsi_deg=10.0, n_times=20), cl_init=np.zeros(6))
+    tb.cl = np.zeros_like(tb.cl)
+    m = March(tb, 9, winds.opts(d=1e-4))
     m.step(12)
     m.sync()
     nodes, _, _ = m.state()
-    from oracle.march import OracleMarch
-    om = OracleMarch(rot, 9)
     p = m.info()["p"]
-    # the node of age a (0-based, a >= 1) left the trailing edge a-1 steps ago
+    U = tb.wind[0]
     for a in range(1, nodes.shape[2]):
-        t_shed = (p - 1 - (a - 1)) * rot.dt
-        _, tr = om.quarter_trail(t_shed)
-        expect = tr + rot.uinf[None, :, None] * (a - 1) * rot.dt
-        assert np.max(np.abs(nodes[:, :, a, :] - expect)) <= 1e-11 * rot.R, a
+        tr = tb.trail[:, :, p - 1 - (a - 1), :]
+        expect = tr + U[None, :, None] * (a - 1) * tb.dt
+        assert np.max(np.abs(nodes[:, :, a, :] - expect)) <= 1e-11 * tb.tip_radius, a
+
+
+def test_tables_end_is_an_error(need_gpu):
+    from winds.march import March
+    m = March(small_tables(10), 9, winds.opts(d=1e-4))
+    m.step(8)                     # time indices 2..9 need tables up to 10
+    with pytest.raises(winds.WindsError, match="tables end"):
+        m.step(1)

@@ -17,34 +17,43 @@
   lifting line for an elliptic wing (C_L = 2 pi alpha0 / (1 + 2/AR), uniform
   downwash, elliptic circulation), with first-order convergence in the
   station spacing;
-- perform.m:42-62 against blade-element forces built as vectors (relative
-  speed projected on the inflow direction, lift normal and drag along it).
+- perform.m:16-62 against blade-element forces built as vectors (relative
+  speed projected on the inflow direction, lift normal and drag along it),
+  with the rotor-frame inflow velX of a rigid rotor against its closed form;
+- KuttaJoukowski.m:262-269: each station reads its own airfoil's table.
+
+The oracle reads the same kinematic tables as the device march
+(winds.cases.tables); the tables themselves are pinned in test_kinematics.py.
 """
+import math
+
 import numpy as np
 
 from oracle.march import OracleMarch
-from winds import rotor as rmod
+from winds import cases
 
 
-def small_rotor(**kw):
-    return rmod.synthetic_rotor(nb=2, nst=7, R=20.0, hub=1.0, tsr=6.0, U=10.0, dpsi_deg=10.0,
-                                **kw)
+def small_rotor(n_times=40, **kw):
+    return cases.tables(cases.synthetic_rotor(nb=2, nst=7, R=20.0, hub=1.0, tsr=6.0, U=10.0,
+                                              dpsi_deg=10.0, n_times=n_times, **kw))
 
 
 def test_r0_and_vcore_hand_values():
-    nrel = rmod.synthetic_rotor(nb=3, nst=7, R=63.0, hub=1.5, tsr=7.0, U=11.4)
+    nrel = cases.tables(cases.synthetic_rotor(nb=3, nst=7, R=63.0, hub=1.5, tsr=7.0, U=11.4,
+                                              n_times=8))
     om = OracleMarch(nrel, 6)
-    assert abs(om.r0 - 0.04558) <= 5e-6
+    assert abs(om.r0(1) - 0.04558) <= 5e-6
     dom = om.dom
     A = dom.shape[2]
     gs = np.full((nrel.ns, 1, A, nrel.nb), 60.0)
     gt = np.full((nrel.nst, 1, A - 1, nrel.nb), -60.0)
+    r0 = om.r0(1)
     rcs, rct, _ = om.vcore(1 + 0, dom, gs, gt, None)          # Time 0: r0^2 only
-    assert np.allclose(rcs, om.r0 ** 2, rtol=1e-15) and np.allclose(rct, om.r0 ** 2, rtol=1e-15)
+    assert np.allclose(rcs, r0 ** 2, rtol=1e-15) and np.allclose(rct, r0 ** 2, rtol=1e-15)
     rcs, rct, lens = om.vcore(3, dom, gs, gt, None)           # p <= 3: no strain
     T = 2 * om.dt
-    rate_s = (rcs - om.r0 ** 2) / T
-    rate_t = (rct - om.r0 ** 2) / T
+    rate_s = (rcs - r0 ** 2) / T
+    rate_t = (rct - r0 ** 2) / T
     assert np.allclose(rate_s, 0.0196506, rtol=3e-6) and np.allclose(rate_t, 0.0196506, rtol=3e-6)
 
 
@@ -54,7 +63,7 @@ def test_filamentmod_strain_cases():
         om.step()
     dom = om.dom
     A = dom.shape[2]
-    r = om.r
+    r = om.tb
     gs = np.full((r.ns, 1, A, r.nb), 10.0)
     gt = np.full((r.nst, 1, A - 1, r.nb), 10.0)
     p = 8
@@ -63,8 +72,9 @@ def test_filamentmod_strain_cases():
     # no strain: the previous lengths are these lengths
     rcs, rct, _ = om.vcore(p, dom, gs, gt, lens)
     Tfac = (p - 1) / 2.0
-    base_s = om.r0 ** 2 + (rcs0 - om.r0 ** 2) * Tfac
-    base_t = om.r0 ** 2 + (rct0 - om.r0 ** 2) * Tfac
+    r0 = om.r0(p)
+    base_s = r0 ** 2 + (rcs0 - r0 ** 2) * Tfac
+    base_t = r0 ** 2 + (rct0 - r0 ** 2) * Tfac
     assert np.allclose(rcs, base_s, rtol=1e-14) and np.allclose(rct, base_t, rtol=1e-14)
     # every filament four times as long (about the origin): strain 1
     big = 4.0 * dom
@@ -109,18 +119,18 @@ def test_kelvin_keeps_the_lattice_closed():
 
 
 def test_zero_lift_nodes_move_with_the_free_stream():
-    rot = small_rotor()
-    rot.cl = np.zeros_like(rot.cl)
-    om = OracleMarch(rot, 9)
+    tb = small_rotor()
+    tb.cl = np.zeros_like(tb.cl)
+    tb.cl_init = np.zeros_like(tb.cl_init)
+    om = OracleMarch(tb, 9)
     for _ in range(12):
         om.step()
     p = 2 + 12
     dom = om.dom
     for a in range(1, dom.shape[2]):
-        t_shed = (p - 1 - (a - 1)) * rot.dt
-        _, tr = om.quarter_trail(t_shed)
-        expect = tr + rot.uinf[None, :, None] * (a - 1) * rot.dt
-        assert np.max(np.abs(dom[:, :, a, :] - expect)) <= 1e-11 * rot.R, a
+        tr = tb.trail[:, :, p - 1 - (a - 1), :]
+        expect = tr + tb.wind[0][None, :, None] * (a - 1) * tb.dt
+        assert np.max(np.abs(dom[:, :, a, :] - expect)) <= 1e-11 * tb.tip_radius, a
 
 
 def _rotation_field(omega):
@@ -136,8 +146,8 @@ def _integrate(om, dom, steps, dt, field, pcc):
     om.dt = dt
     for _ in range(steps):
         v = field(dom)
-        pred = om._advect(dom, v, 0.0)
-        dom = om._advect(dom, v, 0.0, vel_next=field(pred)) if pcc else pred
+        pred = om._advect(dom, v, 2)
+        dom = om._advect(dom, v, 2, vel_next=field(pred)) if pcc else pred
     return dom
 
 
@@ -188,7 +198,7 @@ def test_ab4_is_fourth_order():
         dom = dom0.copy()
         for _ in range(k):
             v = field(dom)
-            dom = om._advect(dom, v, 0.0, hist=(v, hist[0], hist[1], hist[2]))
+            dom = om._advect(dom, v, 2, hist=(v, hist[0], hist[1], hist[2]))
             hist = [v, hist[0], hist[1]]
         errs.append(np.max(np.abs(dom[:, :, a0 + k, :] - exact)))
     rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
@@ -196,9 +206,10 @@ def test_ab4_is_fourth_order():
 
 
 def _elliptic_wing(nst, alpha0_deg, hub=5.0, R=65.0, c0=7.5, U=10.0, far=1e6):
-    """A fixed blade (Omega = 0, one blade along +y) with an elliptic chord
-    and a linear polar cl = 2 pi alpha, set at geometric angle of attack
-    alpha0 in a uniform stream U e_x, with a flat wake of semi-infinite
+    """A fixed wing along +y with an elliptic chord and a linear polar
+    cl = 2 pi alpha, its chord along e_x pitched nose-up by alpha0 (blade
+    frame bxn = cos(a0) e_x - sin(a0) e_z, bzn = e_y, byn = bzn x bxn, so the
+    bare stream U e_x meets it at alpha0), in a flat wake of semi-infinite
     trailing lines: one wake age far downstream (A = 2), so the Kelvin update
     of KuttaJoukowski.m:285-288 moves every trailing filament with the bound
     circulation and kj() alone solves the steady lifting line.  The MEX
@@ -208,19 +219,27 @@ def _elliptic_wing(nst, alpha0_deg, hub=5.0, R=65.0, c0=7.5, U=10.0, far=1e6):
 
     r = gen.cosine_stations(nst, R, hub)
     rn = r[:-1] + np.diff(r) / 2
+    ns = nst - 1
     eta = (2 * rn - R - hub) / (R - hub)
     cn = c0 * np.sqrt(1 - eta ** 2)
     a = np.linspace(-20.0, 20.0, 81)
-    # twist + pitch of 90 deg lays the chord along e_x; 90 - alpha0 pitches
-    # the nose up by alpha0 (aoa of the bare stream = alpha0)
-    blade = SimpleNamespace(r_node=rn, twist_node=np.full(rn.size, 90.0 - alpha0_deg),
-                            pitch_deg=0.0, chord_node=cn, alpha=a,
-                            cl=2 * np.pi * np.radians(a), cd=np.zeros_like(a))
-    om = object.__new__(OracleMarch)
-    om.r, om.nb, om.nst, om.ns, om.om = blade, 1, nst, nst - 1, 0.0
-    om.U = np.array([U, 0.0, 0.0])
-    om.d, om.co, om.sem, om.threads = 1e-12, np.inf, "mex", 1
-    om.relax, om.tol, om.maxiter = 0.25, 1e-13, 2000
+    a0 = np.radians(alpha0_deg)
+    bx = np.array([np.cos(a0), 0.0, -np.sin(a0)])
+    bz = np.array([0.0, 1.0, 0.0])
+    by = np.cross(bz, bx)
+    Uv = np.array([U, 0.0, 0.0])
+    tile = lambda v: np.asfortranarray(np.broadcast_to(v[None, :, None, None], (ns, 3, 1, 1)))
+    bound = np.zeros((ns, 3, 1, 1), order="F")
+    bound[:, 1, 0, 0] = rn
+    tb = SimpleNamespace(nb=1, nst=nst, ns=ns, n_times=1, tip_radius=R, dt=1.0,
+                         chord_node=cn, twist_node=np.zeros(ns), dr_node=np.diff(r), r_node=rn,
+                         foil=np.ones(ns, dtype=np.int32), foil_len=np.array([a.size]),
+                         alpha=a, cl=2 * np.pi * np.radians(a), cd=np.zeros_like(a),
+                         bound=bound, bxn=tile(bx), byn=tile(by), bzn=tile(bz),
+                         vblade=tile(np.array([bx @ Uv, by @ Uv, bz @ Uv])), ll_extra=None,
+                         wind=np.array([[U, 0.0, 0.0]]))
+    om = OracleMarch(tb, 2, d=1e-12, semantics="mex", relax=0.25, tol=1e-13, maxiter=2000,
+                     threads=1, setup=False)
     dom = np.zeros((nst, 3, 2, 1))
     dom[:, 1, :, 0] = r[:, None]
     dom[:, 0, 1, 0] = far
@@ -228,7 +247,7 @@ def _elliptic_wing(nst, alpha0_deg, hub=5.0, R=65.0, c0=7.5, U=10.0, far=1e6):
     g0 = np.zeros((nst - 1, 1))
     om.gs, om.gt = OracleMarch.kelvin(g0, np.zeros((nst, 1, 0, 1)), 2)
     om.gbound = g0
-    iters, cl, _, aoa = om.kj(0.0, np.ones((nst - 1, 1, 2, 1)), np.ones((nst, 1, 1, 1)))
+    iters, cl, _, aoa = om.kj(1, np.ones((nst - 1, 1, 2, 1)), np.ones((nst, 1, 1, 1)))
     assert iters < om.maxiter
     b = R - hub
     S = np.pi * b * c0 / 4
@@ -269,39 +288,69 @@ def test_kj_matches_prandtl_elliptic_wing():
 
 
 def test_perform_resolves_blade_element_forces():
-    """perform.m:42-62 against the blade-element forces built as vectors:
-    the relative speed is the projection of the kinematic velocity on the
-    inflow direction n_D = sin(phi) e_x - cos(phi) e_t (the induction factors
-    of perform.m make the induced velocity normal to it: v1 (1 - a) =
-    W sin(phi), v2 (1 + a') = W cos(phi)), drag acts along n_D and lift along
-    n_L = cos(phi) e_x + sin(phi) e_t, thrust is the e_x component of the sum
-    and torque the sum of r times the e_t components."""
-    rot = small_rotor()
-    om = OracleMarch(rot, 6)
+    """perform.m:16-62 against the blade-element forces built as vectors.
+    For a rigid rotor about +x the rotor-frame inflow velX of perform.m:19-24
+    is (axial speed, in-plane speed Omega r, 0) up to ctdiff's truncation;
+    the relative speed is the projection of the kinematic velocity
+    V = U e_x - Omega r e_t on the inflow direction n_D = sin(phi) e_x -
+    cos(phi) e_t (the induction factors of perform.m make the induced velocity
+    normal to it: v1 (1 - a) = W sin(phi), v2 (1 + a') = W cos(phi)), drag
+    acts along n_D and lift along n_L = cos(phi) e_x + sin(phi) e_t, thrust is
+    the e_x component of the sum and torque the sum of r times the e_t
+    components."""
+    tb = small_rotor()
+    om = OracleMarch(tb, 6)
     rng = np.random.default_rng(11)
-    ns, nb = rot.ns, rot.nb
+    ns, nb = tb.ns, tb.nb
     aoa = rng.uniform(2.0, 14.0, (ns, nb))
     cl = rng.uniform(0.3, 1.4, (ns, nb))
     cd = rng.uniform(0.005, 0.05, (ns, nb))
-    t = 0.37
-    T, Q, Pw = om.perform(cl, cd, aoa, t)
-    psi = 2 * np.pi * np.arange(nb) / nb + rot.omega * t
-    ex = np.array([1.0, 0.0, 0.0])
+    p = 9
+    U = tb.wind[0][0]
+    omega = tb.rot_speed[0] * math.pi / 30
+    vX = om.velX(p)
+    assert np.allclose(vX[:, 0, :], U, rtol=2e-4)
+    assert np.allclose(vX[:, 1, :], omega * tb.r_node[:, None], rtol=2e-4)
+    assert np.all(np.abs(vX[:, 2, :]) <= 1e-9 * U)
+    T, Q, Pw = om.perform(cl, cd, aoa, p)
+    ex = np.array([1.0, 0.0])          # (axial, tangential) plane of the section
+    et = np.array([0.0, 1.0])
     Tv = Qv = 0.0
     for b in range(nb):
-        et = np.array([0.0, -np.sin(psi[b]), np.cos(psi[b])])
+        pitch = tb.pitch[p - 1, b]
         for i in range(ns):
-            rn = rot.r_node[i]
-            V = rot.uinf - rot.omega * rn * et
-            phi = np.radians(aoa[i, b] - (rot.pitch_deg + rot.twist_node[i]))
+            rn = tb.r_node[i]
+            V = vX[i, 0, b] * ex - vX[i, 1, b] * et
+            phi = np.radians(aoa[i, b] - (pitch + tb.twist_node[i]))
             nD = np.sin(phi) * ex - np.cos(phi) * et
             nL = np.cos(phi) * ex + np.sin(phi) * et
             W = V @ nD
-            F = 0.5 * 1.23 * rot.chord_node[i] * W * W * rot.dr_node[i] * (cl[i, b] * nL + cd[i, b] * nD)
+            F = 0.5 * 1.23 * tb.chord_node[i] * W * W * tb.dr_node[i] * (cl[i, b] * nL + cd[i, b] * nD)
             Tv += F @ ex
             Qv += rn * (F @ et)
     assert abs(T - Tv) <= 1e-12 * abs(Tv)
     assert abs(Q - Qv) <= 1e-12 * abs(Qv)
-    assert abs(Pw - rot.omega * Qv) <= 1e-12 * abs(rot.omega * Qv)
-    # an axial rotor's loads do not depend on its azimuth
-    assert abs(om.perform(cl, cd, aoa, t + 0.9)[0] - T) <= 1e-12 * abs(T)
+    assert abs(Pw - omega * Qv) <= 1e-12 * abs(omega * Qv)
+    # an axial rotor's loads do not depend on its azimuth (interior indices)
+    assert abs(om.perform(cl, cd, aoa, p + 7)[0] - T) <= 1e-6 * abs(T)
+
+
+def test_kj_uses_each_stations_airfoil():
+    """KuttaJoukowski.m:262-269: station m interpolates table blade.NFoil(m).
+    Three tables whose lift differs by a constant factor; the stations on
+    each table get exactly that table's cl at their angle of attack."""
+    tb = small_rotor()
+    a = tb.alpha.copy()
+    base = tb.cl.copy()
+    n = a.size
+    tb.alpha = np.concatenate([a, a, a])
+    tb.cl = np.concatenate([base, 1.5 * base, 0.5 * base])
+    tb.cd = np.concatenate([tb.cd, tb.cd, tb.cd])
+    tb.foil_len = np.array([n, n, n], dtype=np.int32)
+    tb.foil = np.array([1, 2, 3, 1, 2, 3], dtype=np.int32)
+    om = OracleMarch(tb, 6, setup=False)
+    aoa = np.linspace(2.0, 9.0, 12).reshape(6, 2)
+    got = om.polar(tb.cl, aoa)
+    ref = np.interp(aoa, a, base)
+    fac = np.array([1.0, 1.5, 0.5, 1.0, 1.5, 0.5])[:, None]
+    assert np.allclose(got, fac * ref, rtol=1e-15)

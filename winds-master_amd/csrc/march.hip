@@ -35,89 +35,67 @@ constexpr double kA1 = 6.5e-5;        // :25
 constexpr double kRho = 1.23;         // :28
 constexpr double kPi = 3.141592653589793;
 
+// Rotor description on the device.  Blade data per station; the polars of
+// every airfoil concatenated (table k: entries [foil_off[k], +foil_len[k]));
+// kinematics per time index in the reference's column-major layouts
+// (pos.quarter/trail [nst,3,nt,nb], pos.bound, pos.nodes.bxn/byn/bzn,
+// vel.blade [ns,3,nt,nb], blade pitch [nt,nb]).
 struct Geom {
-    const double *r_trail, *chord_trail, *twist_trail;
     const double *r_node, *chord_node, *twist_node, *dr_node;
-    const double *alpha, *cl, *cd;
-    int nb, nst, ns, n_alpha;
-    double omega, pitch_deg, ux, uy, uz;
+    const int *foil;                  // table of station m (0-based)
+    const int *foil_off, *foil_len;   // per table
+    const double *alpha, *cl, *cd;    // concatenated tables
+    const double *quarter, *trail;
+    const double *bound, *bxn, *byn, *bzn, *vblade;
+    const double *ll_extra;           // induced velocity of other sources at the
+                                      // lifting line, or null
+    const double *pitch, *rot_speed;
+    int nb, nst, ns, nt, n_foils, n_alpha;
 };
 
 __device__ __forceinline__ double rad(double deg) { return deg * kPi / 180.0; }
 
-// e_r, e_t of blade b at time t (azimuth 2 pi b/nb + omega t)
-__device__ __forceinline__ void frame(const Geom &g, int b, double t, double &cp, double &sp) {
-    const double psi = 2.0 * kPi * b / g.nb + g.omega * t;
-    cp = cos(psi);
-    sp = sin(psi);
+// element (i, c, p0, b) of a column-major [n,3,nt,nb] table, p0 0-based
+__device__ __forceinline__ double tab4(const double *t, int n, int nt, int i, int c, int p0,
+                                       int b) {
+    return t[i + (int64_t)n * (c + 3 * (p0 + (int64_t)nt * b))];
 }
 
-// chord direction e_c = -cos(th) e_t + sin(th) e_x, th = twist + pitch
-__device__ __forceinline__ void chord_dir(double cp, double sp, double th_deg, double &cx,
-                                          double &cy, double &cz) {
-    const double th = rad(th_deg);
-    const double ct = cos(th), st = sin(th);
-    cx = st;
-    cy = -ct * -sp;
-    cz = -ct * cp;
-}
-
-// quarter-chord and trailing-edge nodes (ages 0 and 1) of a wake of A ages
-__global__ void k_nodes01(Geom g, double t, int A, double *__restrict__ X) {
+// wake ages 0 and 1 of time index p0+1: pos.quarter and pos.trail
+// (initials.m:87-88, kinematics.m:49-50)
+__global__ void k_nodes01(Geom g, int p0, int A, double *__restrict__ X) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.nst * g.nb) return;
     const int m = i % g.nst, b = i / g.nst;
     const int64_t N = (int64_t)g.nst * A * g.nb;
-    double cp, sp;
-    frame(g, b, t, cp, sp);
-    const double r = g.r_trail[m];
-    const double qx = r * 0.0, qy = r * cp, qz = r * sp;
-    double ex, ey, ez;
-    chord_dir(cp, sp, g.twist_trail[m] + g.pitch_deg, ex, ey, ez);
-    const double c = 0.75 * g.chord_trail[m];
     const int64_t j0 = m + (int64_t)b * g.nst * A;
     const int64_t j1 = j0 + g.nst;
-    X[j0] = qx;
-    X[N + j0] = qy;
-    X[2 * N + j0] = qz;
-    X[j1] = qx + c * ex;
-    X[N + j1] = qy + c * ey;
-    X[2 * N + j1] = qz + c * ez;
+    for (int c = 0; c < 3; ++c) {
+        X[c * N + j0] = tab4(g.quarter, g.nst, g.nt, m, c, p0, b);
+        X[c * N + j1] = tab4(g.trail, g.nst, g.nt, m, c, p0, b);
+    }
 }
 
-// lifting-line frame at station m of blade b: bound point, bxn/byn/bzn and
-// the inflow U - Omega r e_t seen in the blade frame (velocity.m:58-70)
+// lifting-line point of station m of blade b at time index p0+1: bound
+// point, blade frame bxn/byn/bzn and the kinematic inflow vel.blade
+// (velocity.m:43-70)
 struct LL {
     double px, py, pz;
     double bx[3], by[3], bz[3];
-    double vb[3];      // inertial inflow
-    double vblade[3];  // blade-frame inflow
+    double vblade[3];
 };
 
-__device__ __forceinline__ LL ll_frame(const Geom &g, int m, int b, double t) {
+__device__ __forceinline__ LL ll_frame(const Geom &g, int m, int b, int p0) {
     LL L;
-    double cp, sp;
-    frame(g, b, t, cp, sp);
-    const double r = g.r_node[m];
-    L.px = r * 0.0;
-    L.py = r * cp;
-    L.pz = r * sp;
-    chord_dir(cp, sp, g.twist_node[m] + g.pitch_deg, L.bx[0], L.bx[1], L.bx[2]);
-    L.bz[0] = 0.0;
-    L.bz[1] = cp;
-    L.bz[2] = sp;
-    // byn = bzn x bxn
-    L.by[0] = L.bz[1] * L.bx[2] - L.bz[2] * L.bx[1];
-    L.by[1] = L.bz[2] * L.bx[0] - L.bz[0] * L.bx[2];
-    L.by[2] = L.bz[0] * L.bx[1] - L.bz[1] * L.bx[0];
-    const double etx = 0.0, ety = -sp, etz = cp;
-    const double w = g.omega * r;
-    L.vb[0] = g.ux - w * etx;
-    L.vb[1] = g.uy - w * ety;
-    L.vb[2] = g.uz - w * etz;
-    L.vblade[0] = L.bx[0] * L.vb[0] + L.bx[1] * L.vb[1] + L.bx[2] * L.vb[2];
-    L.vblade[1] = L.by[0] * L.vb[0] + L.by[1] * L.vb[1] + L.by[2] * L.vb[2];
-    L.vblade[2] = L.bz[0] * L.vb[0] + L.bz[1] * L.vb[1] + L.bz[2] * L.vb[2];
+    L.px = tab4(g.bound, g.ns, g.nt, m, 0, p0, b);
+    L.py = tab4(g.bound, g.ns, g.nt, m, 1, p0, b);
+    L.pz = tab4(g.bound, g.ns, g.nt, m, 2, p0, b);
+    for (int c = 0; c < 3; ++c) {
+        L.bx[c] = tab4(g.bxn, g.ns, g.nt, m, c, p0, b);
+        L.by[c] = tab4(g.byn, g.ns, g.nt, m, c, p0, b);
+        L.bz[c] = tab4(g.bzn, g.ns, g.nt, m, c, p0, b);
+        L.vblade[c] = tab4(g.vblade, g.ns, g.nt, m, c, p0, b);
+    }
     return L;
 }
 
@@ -162,15 +140,15 @@ __device__ __forceinline__ void interp1x2(const double *xp, const double *fa, co
     yb = (fb[lo + 1] - fb[lo]) / dx * (x - xp[lo]) + fb[lo];
 }
 
-// initials.m:110-125: bound circulation 0.5*|U|*chord*cl(geometric aoa)
-__global__ void k_init_bound(Geom g, double t, double umag, int A, double *__restrict__ gs) {
+// initials.m:110-125: bound circulation 0.5*|U(1)|*chord*cl, cl the
+// initial lift coefficient of each station (BEM.m or the elliptic wing's
+// closed form, initials.m:111-119), the same on every blade
+__global__ void k_init_bound(Geom g, double umag, const double *__restrict__ cl0, int A,
+                             double *__restrict__ gs) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.ns * g.nb) return;
     const int m = i % g.ns, b = i / g.ns;
-    const LL L = ll_frame(g, m, b, t);
-    const double aoa = atan2(-L.vblade[1], L.vblade[0]) * (180.0 / kPi);
-    const double cl0 = interp1(g.alpha, g.cl, g.n_alpha, aoa);
-    gs[m + (int64_t)b * g.ns * A] = 0.5 * umag * g.chord_node[m] * cl0;
+    gs[m + (int64_t)b * g.ns * A] = 0.5 * umag * g.chord_node[m] * cl0[m];
 }
 
 // Kelvin (WInDS.m:131-140), trailing part: age 0 = diff([0; bound; 0]),
@@ -477,11 +455,11 @@ __global__ void k_advect(int nst, int nb, int A_old, int A_new, double dt,
     }
 }
 
-__global__ void k_ll_points(Geom g, double t, double *__restrict__ P) {
+__global__ void k_ll_points(Geom g, int p0, double *__restrict__ P) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = g.ns * g.nb;
     if (i >= n) return;
-    const LL L = ll_frame(g, i % g.ns, i / g.ns, t);
+    const LL L = ll_frame(g, i % g.ns, i / g.ns, p0);
     P[i] = L.px;
     P[n + i] = L.py;
     P[2 * n + i] = L.pz;
@@ -538,14 +516,18 @@ constexpr int kKjThreads = 1024;
 constexpr int kKjWaves = kKjThreads / 64;
 constexpr int kMaxLL = 512;      // lifting-line points handled by one workgroup
 constexpr int kMaxNodesBlade = 256;
-constexpr int kMaxAlpha = 1024;  // polar table entries held in LDS by the KJ kernel
+constexpr int kMaxAlpha = 1024;  // polar table entries (all tables) held in LDS by the KJ kernel
+constexpr int kMaxFoils = 64;    // airfoil tables
 
 // Kutta-Joukowski fixed point (KuttaJoukowski.m:42-49 with kj() :204-296)
-// for time t, in FP64, in one workgroup, followed by perform.m:42-62.
-// frozen: induced velocity at the lifting line of every filament except the
-// changing ones; K their unit influence.  Writes the bound circulation, the
-// trailing age 0 and shed age 1 circulation, cl/cd/aoa and one history row.
-__global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, double relax,
+// for time index p0+1, in FP64, in one workgroup, followed by
+// perform.m:16-62.  frozen: induced velocity at the lifting line of every
+// filament except the changing ones; K their unit influence; other sources
+// (the tower, KuttaJoukowski.m:232-238) come from g.ll_extra.  Each station
+// reads the polar of its airfoil (blade.NFoil, KuttaJoukowski.m:262-269).
+// Writes the bound circulation, the trailing age 0 and shed age 1
+// circulation, cl/cd/aoa and one history row.
+__global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double relax,
                                                   double tol, int maxiter,
                                                   const double *__restrict__ frozen,
                                                   const double *__restrict__ K,
@@ -563,9 +545,10 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
     __shared__ double s_gam[3 * kMaxLL + 4 * kMaxNodesBlade];  // changing circulation by s
     __shared__ double s_c1[kMaxLL];     // span cumsum of trailing age 1
     __shared__ double s_u[3 * kMaxLL];  // induced velocity at the lifting line
-    __shared__ double s_red[kKjThreads];
-    __shared__ double s_red2[kKjThreads];
-    __shared__ double s_al[kMaxAlpha], s_tcl[kMaxAlpha], s_tcd[kMaxAlpha];  // the polar
+    __shared__ double s_red[kKjWaves];
+    __shared__ double s_red2[kKjWaves];
+    __shared__ double s_al[kMaxAlpha], s_tcl[kMaxAlpha], s_tcd[kMaxAlpha];  // the polars
+    __shared__ int s_foff[kMaxFoils], s_flen[kMaxFoils];
     __shared__ double s_fz[3 * kMaxLL];   // the frozen filaments' velocity at the points
     __shared__ double s_ll[10 * kMaxLL];  // per point: chord frame, inflow, |inflow|, chord
     __shared__ int s_nan;
@@ -579,13 +562,17 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
         s_tcl[i] = g.cl[i];
         s_tcd[i] = g.cd[i];
     }
+    for (int i = tid; i < g.n_foils; i += kKjThreads) {
+        s_foff[i] = g.foil_off[i];
+        s_flen[i] = g.foil_len[i];
+    }
     // Thread i < NLL owns lifting-line point i (NLL <= kMaxLL < kKjThreads).
     // Its frame, inflow, chord and frozen velocity do not change during the
     // iteration: computed once, kept in LDS (in registers they pushed the
     // kernel past its 128 VGPRs into spills).
     if (tid < NLL) {
         const int m = tid % ns, b = tid / ns;
-        const LL L = ll_frame(g, m, b, t);
+        const LL L = ll_frame(g, m, b, p0);
         double *q = s_ll + 10 * tid;
         q[0] = L.bx[0];
         q[1] = L.bx[1];
@@ -598,9 +585,15 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
         q[8] = sqrt(L.vblade[0] * L.vblade[0] + L.vblade[1] * L.vblade[1] +
                     L.vblade[2] * L.vblade[2]);
         q[9] = g.chord_node[m];
-        s_fz[tid] = frozen[tid];
-        s_fz[NLL + tid] = frozen[NLL + tid];
-        s_fz[2 * NLL + tid] = frozen[2 * NLL + tid];
+        double ex = 0.0, ey = 0.0, ez = 0.0;
+        if (g.ll_extra) {
+            ex = tab4(g.ll_extra, ns, g.nt, m, 0, p0, b);
+            ey = tab4(g.ll_extra, ns, g.nt, m, 1, p0, b);
+            ez = tab4(g.ll_extra, ns, g.nt, m, 2, p0, b);
+        }
+        s_fz[tid] = frozen[tid] + ex;
+        s_fz[NLL + tid] = frozen[NLL + tid] + ey;
+        s_fz[2 * NLL + tid] = frozen[2 * NLL + tid] + ez;
     }
     if (tid < nb) {
         const int b = tid;
@@ -685,7 +678,9 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
             const double Vinf = q[8], chord = q[9];
             const double aoa = atan2(-tv, tu) * (180.0 / kPi);
             double cl, cd;
-            interp1x2(s_al, s_tcl, s_tcd, g.n_alpha, aoa, cl, cd);
+            const int f = g.foil[i % ns];
+            const int o = s_foff[f];
+            interp1x2(s_al + o, s_tcl + o, s_tcd + o, s_flen[f], aoa, cl, cd);
             if (cl != cl) s_nan = 1;
             const double gn = 0.5 * Vinf * chord * cl;
             const double dg = gn - s_g[i];
@@ -725,21 +720,21 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
         }
     }
     __syncthreads();
-    // perform.m:42-62 from this time's cl, cd, aoa
+    // perform.m:16-62 from this time's cl, cd, aoa: velX = the blade-frame
+    // inflow vel.blade turned back by twist and pitch (DCMRot with
+    // [-90 AeroTwst BldPitch] about z, negated; perform.m:19-24)
     double dT = 0.0, dQ = 0.0;
     for (int i = tid; i < NLL; i += kKjThreads) {
         const int m = i % ns, b = i / ns;
-        double cp, sp;
-        frame(g, b, t, cp, sp);
-        const double r = g.r_node[m];
-        const double etx = 0.0, ety = -sp, etz = cp;
-        const double vbx = g.ux - g.omega * r * etx;
-        const double vby = g.uy - g.omega * r * ety;
-        const double vbz = g.uz - g.omega * r * etz;
-        const double v1 = vbx;
-        const double v2 = -(vbx * etx + vby * ety + vbz * etz);
+        const double pitch = g.pitch[p0 + (int64_t)g.nt * b];
+        const double th = rad(-90.0 + g.twist_node[m] + pitch);
+        const double ct = cos(th), st = sin(th);
+        const double vb0 = tab4(g.vblade, ns, g.nt, m, 0, p0, b);
+        const double vb1 = tab4(g.vblade, ns, g.nt, m, 1, p0, b);
+        const double v1 = -(ct * vb0 - st * vb1);
+        const double v2 = -(st * vb0 + ct * vb1);
         const double tsr = v2 / v1;
-        const double phi = aoa_out[i] - (g.pitch_deg + g.twist_node[m]);
+        const double phi = aoa_out[i] - (pitch + g.twist_node[m]);
         const double tphi = tan(rad(phi));
         const double aap = tsr / tphi;
         const double a = aap * (v1 - tphi * v2) / (aap * v1 + tphi * v2);
@@ -751,23 +746,27 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, double t, int A, doub
         const double W2 = w1 * w1 + w2 * w2;
         const double c = g.chord_node[m];
         dT += 0.5 * kRho * c * dCZ * W2 * g.dr_node[m];
-        dQ += 0.5 * kRho * c * dCX * W2 * r * g.dr_node[m];
+        dQ += 0.5 * kRho * c * dCX * W2 * g.r_node[m] * g.dr_node[m];
     }
-    s_red[tid] = dT;
-    s_red2[tid] = dQ;
+    for (int off = 32; off > 0; off >>= 1) {
+        dT += __shfl_xor(dT, off);
+        dQ += __shfl_xor(dQ, off);
+    }
+    if ((tid & 63) == 0) {
+        s_red[tid >> 6] = dT;
+        s_red2[tid >> 6] = dQ;
+    }
     __syncthreads();
-    for (int w = kKjThreads / 2; w > 0; w >>= 1) {
-        if (tid < w) {
-            s_red[tid] += s_red[tid + w];
-            s_red2[tid] += s_red2[tid + w];
-        }
-        __syncthreads();
-    }
     if (tid == 0) {
+        double T = 0.0, Q = 0.0;
+        for (int w = 0; w < kKjWaves; ++w) {
+            T += s_red[w];
+            Q += s_red2[w];
+        }
         hist_row[0] = p_index;
-        hist_row[1] = s_red[0];
-        hist_row[2] = s_red2[0];
-        hist_row[3] = g.omega * s_red2[0];
+        hist_row[1] = T;
+        hist_row[2] = Q;
+        hist_row[3] = (kPi / 30.0) * g.rot_speed[p0] * Q;  // perform.m:52-56
         hist_row[4] = (double)j;
         if (s_nan) *flag = 1;
     }
@@ -813,7 +812,9 @@ struct winds_march {
     int A_gprev = 1;  // ages of gs/gt[gprev]
     int lcur = 0;     // ls/lt[lcur]: filament lengths of time p-1
     int nsteps_done = 0;
-    double r0 = 0.0;
+    // per time index on the host (index p-1): fastout.Time, wind.infty,
+    // fastout.TipSpdRat, fastout.Azimuth, pos.hub
+    std::vector<double> h_time, h_wind, h_tsr, h_azi, h_hub;
     DBuf geom, nodes[2], vel, velp, uind, hist[3];
     int hist_A[3] = {0, 0, 0};
     int n_hist = 0;
@@ -821,13 +822,13 @@ struct winds_march {
     int ls_A[3] = {0, 0, 0};
     DBuf segs, ws, llp, frozen, K, kjout, perf, flag;
     // far-wake surrogate (winds_march_set_surrogate); net is not owned
-    double umag = 0.0;
     winds_mlp *net = nullptr;
     winds_march_surr sc{};
     bool surr = false;
     bool fallback = true;  // use the direct far sum until a check passes
     int A_ref = 0;         // ages of the turning-frame store ufz (0: empty)
     int64_t since_check = 0;
+    const double *cl0 = nullptr;  // initial lift coefficient per station (device)
     DBuf norm, tgt, unear, ufar, umlp, ufz, llfar, llmlp, feat, lab, yout, desc, err;
     double stats[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
@@ -918,12 +919,25 @@ int kelvin(winds_march *m, const double *gs_old, int A_old, const double *gt_old
     return WINDS_OK;
 }
 
+double umag_of(const winds_march *m, int p) {
+    const double *w = m->h_wind.data();
+    const int nt = m->cfg.n_times;
+    const double x = w[p - 1], y = w[nt + p - 1], z = w[2 * nt + p - 1];
+    return sqrt(x * x + y * y + z * z);
+}
+
+// initials.m:134-135: r0(p) = sqrt(4 alpha nu delta T0), T0 = 2 pi R / (12 TSR |U|)
+double r0_of(const winds_march *m, int p) {
+    const double T0 = 2 * kPi * m->cfg.tip_radius / (12 * m->h_tsr[p - 1] * umag_of(m, p));
+    return sqrt(4 * kAlphaRL * kNu * kDelta * T0);
+}
+
 int vcore(winds_march *m, int p, const double *X, int A, const double *gsp, const double *gtp,
           int lprev, int lout, double *rsp, double *rtp) {
-    const double T = (p - 1) * m->cfg.dt;
+    const double T = m->h_time[p - 1];  // vcore.m:34-37: fastout.Time(time)
     const int fmod = p > 3 ? 1 : 0;
     const int64_t n = segs_of(m, A);
-    k_vcore<<<blocks(n), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, X, gsp, gtp, m->r0, T, fmod,
+    k_vcore<<<blocks(n), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, X, gsp, gtp, r0_of(m, p), T, fmod,
                                           m->ls[lprev].d(), m->lt[lprev].d(),
                                           std::max(2, m->ls_A[lprev]), m->ls[lout].d(),
                                           m->lt[lout].d(), rsp, rtp);
@@ -933,8 +947,8 @@ int vcore(winds_march *m, int p, const double *X, int A, const double *gsp, cons
 }
 
 int advect(winds_march *m, const double *Xo, int A_old, const double *vo, const double *vpred,
-           bool ab4, double t_next, double *Xn, int A_new) {
-    k_nodes01<<<blocks((int64_t)m->g.nst * m->g.nb), 256, 0, m->st>>>(m->g, t_next, A_new, Xn);
+           bool ab4, int p_next, double *Xn, int A_new) {
+    k_nodes01<<<blocks((int64_t)m->g.nst * m->g.nb), 256, 0, m->st>>>(m->g, p_next - 1, A_new, Xn);
     WM_HIP(hipGetLastError());
     const int64_t n = (int64_t)m->g.nst * std::max(0, A_new - 2) * m->g.nb;
     if (n > 0) {
@@ -970,11 +984,17 @@ int push_hist(winds_march *m, const double *vel, int A) {
 
 bool surr_on(const winds_march *m, int A) { return m->surr && A > m->sc.split; }
 
-double psi_of(const winds_march *m, double t) { return m->cfg.omega * t; }
+// Angle of the frame turning with the rotor at time index p: the azimuth of
+// blade 0 (fastout.Azimuth) about the rotor axis, offset so that blade 0
+// points along +y at psi = 0 (kinematics.m: blade b lies along z before the
+// rotation by its azimuth about x).
+double psi_of(const winds_march *m, int p) { return m->h_azi[p - 1] * kPi / 180.0 + 0.5 * kPi; }
 
-winds::surr::Geo geo_of(const winds_march *m, int A, double t) {
-    return winds::surr::Geo{m->g.nb, m->g.nst, A, m->sc.split, m->cfg.max_ages, m->cfg.R,
-                            m->umag, psi_of(m, t)};
+winds::surr::Geo geo_of(const winds_march *m, int A, int p) {
+    const int nt = m->cfg.n_times;
+    const double *h = m->h_hub.data();
+    return winds::surr::Geo{m->g.nb, m->g.nst, A, m->sc.split, m->cfg.max_ages, m->cfg.tip_radius,
+                            umag_of(m, p), psi_of(m, p), {h[p - 1], h[nt + p - 1], h[2 * nt + p - 1]}};
 }
 
 int64_t near_targets(const winds_march *m) {
@@ -982,7 +1002,7 @@ int64_t near_targets(const winds_march *m) {
 }
 
 // feat rows [row0, row0+np) for the points P; network output rows likewise
-int mlp_features(winds_march *m, const double *X, int A, const double *gtp, double t,
+int mlp_features(winds_march *m, const double *X, int A, const double *gtp, int t,
                  const double *P, int64_t np, int64_t row0) {
     const float *nin = (const float *)m->norm.p;
     WM_HIP(winds::surr::launch_features(geo_of(m, A, t), X, gtp, P, np, nin, (float *)m->desc.p,
@@ -990,7 +1010,7 @@ int mlp_features(winds_march *m, const double *X, int A, const double *gtp, doub
     return WINDS_OK;
 }
 
-int mlp_apply(winds_march *m, int A, double t, int64_t np, int64_t row0, double *u3) {
+int mlp_apply(winds_march *m, int A, int t, int64_t np, int64_t row0, double *u3) {
     const float *nout = (const float *)m->norm.p + 2 * winds::surr::kNIn;
     WM_HIP(winds::surr::launch_apply(geo_of(m, A, t), (const float *)m->yout.p + row0 * 3, np,
                                      nout, u3, m->st));
@@ -999,7 +1019,7 @@ int mlp_apply(winds_march *m, int A, double t, int64_t np, int64_t row0, double 
 
 // u3 += the far wake's induced velocity at P by the configured method
 int far_part(winds_march *m, const double *X, int A, const double *gsp, const double *gtp,
-             const double *rsp, const double *rtp, double t, const double *P, int64_t np,
+             const double *rsp, const double *rtp, int t, const double *P, int64_t np,
              double *u3, const winds::EvalCfg &c, double *scratch) {
     if (m->sc.mode == WINDS_SURR_DROP || np == 0) return WINDS_OK;
     if (m->sc.mode == WINDS_SURR_MLP && !m->fallback) {
@@ -1021,13 +1041,13 @@ int far_part(winds_march *m, const double *X, int A, const double *gsp, const do
 // the fallback decision and the online training step.  u is the direct
 // induced velocity of the whole wake at its nodes (this check's step).
 int surr_check(winds_march *m, const double *X, int A, const double *gsp, const double *gtp,
-               const double *rsp, const double *rtp, double t, const double *u) {
+               const double *rsp, const double *rtp, int t, const double *u) {
     const int St = m->sc.target_ages;
     const int64_t Nt = near_targets(m);
     const int NLL = m->g.ns * m->g.nb;
     k_gather_near<<<blocks(Nt), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, St, X, m->tgt.d());
     WM_HIP(hipGetLastError());
-    k_ll_points<<<blocks(NLL), 256, 0, m->st>>>(m->g, t, m->llp.d());
+    k_ll_points<<<blocks(NLL), 256, 0, m->st>>>(m->g, t - 1, m->llp.d());
     WM_HIP(hipGetLastError());
     m->stats[0] += 1;
     if (m->sc.mode != WINDS_SURR_MLP) return WINDS_OK;
@@ -1092,7 +1112,7 @@ enum How { kDirect = 0, kDirectCheck = 1, kSurrogate = 2 };
 
 // Induced velocity of the wake at its own nodes, into u (SoA [3][N]).
 int wake_velocity(winds_march *m, const double *X, int A, const double *gsp, const double *gtp,
-                  const double *rsp, const double *rtp, double t, double *u, How how) {
+                  const double *rsp, const double *rtp, int t, double *u, How how) {
     const int64_t N = nodes_of(m, A);
     if (how != kSurrogate) {
         int rc = induced(m, X, A, gsp, gtp, rsp, rtp, X, N, u, m->wake, 0);
@@ -1119,17 +1139,25 @@ int wake_velocity(winds_march *m, const double *X, int A, const double *gsp, con
     return WINDS_OK;
 }
 
+// Free stream of time index p added to the induced velocity u (vel.domain,
+// initials.m:90 and WInDS.m:160)
+int add_wind(winds_march *m, int p, int64_t N, const double *u, double *v) {
+    const int nt = m->cfg.n_times;
+    const double *w = m->h_wind.data();
+    k_add_uinf<<<blocks(N), 256, 0, m->st>>>(N, w[p - 1], w[nt + p - 1], w[2 * nt + p - 1], u, v);
+    WM_HIP(hipGetLastError());
+    return WINDS_OK;
+}
+
 // initials.m:44-152 (time index 1)
 int setup(winds_march *m) {
     const int A = 2;
     double *X = m->nodes[0].d();
-    k_nodes01<<<blocks((int64_t)m->g.nst * m->g.nb), 256, 0, m->st>>>(m->g, 0.0, A, X);
+    k_nodes01<<<blocks((int64_t)m->g.nst * m->g.nb), 256, 0, m->st>>>(m->g, 0, A, X);
     WM_HIP(hipGetLastError());
-    const double umag = sqrt(m->cfg.uinf[0] * m->cfg.uinf[0] + m->cfg.uinf[1] * m->cfg.uinf[1] +
-                             m->cfg.uinf[2] * m->cfg.uinf[2]);
     // the bound circulation as a one-age shed array feeds the Kelvin step
-    k_init_bound<<<blocks((int64_t)m->g.ns * m->g.nb), 256, 0, m->st>>>(m->g, 0.0, umag, 1,
-                                                                        m->gs1.d());
+    k_init_bound<<<blocks((int64_t)m->g.ns * m->g.nb), 256, 0, m->st>>>(
+        m->g, umag_of(m, 1), m->cl0, 1, m->gs1.d());
     WM_HIP(hipGetLastError());
     int rc = kelvin(m, m->gs1.d(), 1, m->gt1.d(), A, m->gs[0].d(), m->gt[0].d());
     if (rc) return rc;
@@ -1139,13 +1167,12 @@ int setup(winds_march *m) {
     rc = induced(m, X, A, m->gs[0].d(), m->gt[0].d(), m->rs.d(), m->rt.d(), X, N, m->uind.d(),
                  m->wake, 0);
     if (rc) return rc;
-    k_add_uinf<<<blocks(N), 256, 0, m->st>>>(N, m->cfg.uinf[0], m->cfg.uinf[1], m->cfg.uinf[2],
-                                             m->uind.d(), m->vel.d());
-    WM_HIP(hipGetLastError());
+    rc = add_wind(m, 1, N, m->uind.d(), m->vel.d());
+    if (rc) return rc;
     rc = push_hist(m, m->vel.d(), A);
     if (rc) return rc;
     const int A_new = std::min(A + 1, m->cfg.max_ages);
-    rc = advect(m, X, A, m->vel.d(), nullptr, false, m->cfg.dt, m->nodes[1].d(), A_new);
+    rc = advect(m, X, A, m->vel.d(), nullptr, false, 2, m->nodes[1].d(), A_new);
     if (rc) return rc;
     m->cur = 1;
     m->A = A_new;
@@ -1159,7 +1186,12 @@ int setup(winds_march *m) {
 // WInDS.m:127-183, one time index p
 int step_once(winds_march *m) {
     const int p = m->p;
-    const double t = (p - 1) * m->cfg.dt;
+    if (p + 1 > m->cfg.n_times)
+        return set_error(WINDS_E_DIMS, "the kinematic tables end at time index " +
+                                           std::to_string(m->cfg.n_times) +
+                                           "; stepping time index " + std::to_string(p) +
+                                           " needs the next one");
+    const int t = p;
     const int A = m->A;
     const int nc = m->cur, nn = 1 - m->cur;        // nodes: time p / time p+1
     const int gp = m->gprev, gc = 1 - m->gprev;    // circulation: time p-1 / time p
@@ -1177,16 +1209,15 @@ int step_once(winds_march *m) {
     const How how = !sa ? kDirect : (chk ? kDirectCheck : kSurrogate);
     rc = wake_velocity(m, X, A, gs, gt, m->rs.d(), m->rt.d(), t, m->uind.d(), how);
     if (rc) return rc;
-    k_add_uinf<<<blocks(N), 256, 0, m->st>>>(N, m->cfg.uinf[0], m->cfg.uinf[1], m->cfg.uinf[2],
-                                             m->uind.d(), m->vel.d());
-    WM_HIP(hipGetLastError());
+    rc = add_wind(m, p, N, m->uind.d(), m->vel.d());
+    if (rc) return rc;
     const int A_new = std::min(A + 1, m->cfg.max_ages);
     double *Xn = m->nodes[nn].d();
     const int integ = m->cfg.integrator;
     if (integ == WINDS_INTEG_AB4 && p >= 4) {
-        rc = advect(m, X, A, m->vel.d(), nullptr, true, t + m->cfg.dt, Xn, A_new);
+        rc = advect(m, X, A, m->vel.d(), nullptr, true, p + 1, Xn, A_new);
     } else {
-        rc = advect(m, X, A, m->vel.d(), nullptr, false, t + m->cfg.dt, Xn, A_new);
+        rc = advect(m, X, A, m->vel.d(), nullptr, false, p + 1, Xn, A_new);
         if (!rc && integ == WINDS_INTEG_PCC && p > 2) {
             // pcc.m:40-75: Kelvin and vcore for p+1 at the predicted nodes,
             // induced velocity there, trapezoidal corrector
@@ -1197,20 +1228,16 @@ int step_once(winds_march *m) {
             const int64_t Nn = nodes_of(m, A_new);
             if (!rc)
                 rc = wake_velocity(m, Xn, A_new, m->gs1.d(), m->gt1.d(), m->rs1.d(), m->rt1.d(),
-                                   t + m->cfg.dt, m->uind.d(),
+                                   t + 1, m->uind.d(),
                                    how == kSurrogate ? kSurrogate : kDirect);
-            if (!rc) {
-                k_add_uinf<<<blocks(Nn), 256, 0, m->st>>>(Nn, m->cfg.uinf[0], m->cfg.uinf[1],
-                                                          m->cfg.uinf[2], m->uind.d(), m->velp.d());
-                WM_HIP(hipGetLastError());
-                rc = advect(m, X, A, m->vel.d(), m->velp.d(), false, t + m->cfg.dt, Xn, A_new);
-            }
+            if (!rc) rc = add_wind(m, p + 1, Nn, m->uind.d(), m->velp.d());
+            if (!rc) rc = advect(m, X, A, m->vel.d(), m->velp.d(), false, p + 1, Xn, A_new);
         }
     }
     if (rc) return rc;
     // Kutta-Joukowski at time p on the wake of time p (KuttaJoukowski.m:215-219)
     const int NLL = m->g.ns * m->g.nb;
-    k_ll_points<<<blocks(NLL), 256, 0, m->st>>>(m->g, t, m->llp.d());
+    k_ll_points<<<blocks(NLL), 256, 0, m->st>>>(m->g, p - 1, m->llp.d());
     WM_HIP(hipGetLastError());
     if (how == kSurrogate) {
         rc = induced(m, X, A, gs, gt, m->rs.d(), m->rt.d(), m->llp.d(), NLL, m->frozen.d(), m->ll,
@@ -1229,9 +1256,9 @@ int step_once(winds_march *m) {
         m->ll.core == 0 ? 0 : m->ll.vat_n, m->ll.d, m->ll.co, m->K.d());
     WM_HIP(hipGetLastError());
     const int row = m->nsteps_done;
-    if (row >= m->cfg.max_steps) return set_error(WINDS_E_NOMEM, "performance history is full");
+    if (row >= m->cfg.n_times) return set_error(WINDS_E_NOMEM, "performance history is full");
     double *kj = m->kjout.d();
-    k_kj<<<1, kKjThreads, 0, m->st>>>(m->g, t, A, m->cfg.relax, m->cfg.tol, m->cfg.maxiter,
+    k_kj<<<1, kKjThreads, 0, m->st>>>(m->g, p - 1, A, m->cfg.relax, m->cfg.tol, m->cfg.maxiter,
                                        m->frozen.d(), m->K.d(), gs, gt, kj, kj + NLL, kj + 2 * NLL,
                                        m->perf.d() + 5 * (int64_t)row, (double)p, (int *)m->flag.p);
     WM_HIP(hipGetLastError());
@@ -1268,16 +1295,32 @@ int winds_march_create(const winds_march_cfg *cfg, winds_march **out) {
     if (!cfg || !out) return set_error(WINDS_E_ARG, "null argument");
     *out = nullptr;
     const winds_march_cfg &c = *cfg;
-    if (c.nb < 1 || c.nst < 2 || c.max_ages < 3 || c.n_alpha < 2 || c.n_alpha > kMaxAlpha ||
-        c.max_steps < 1)
+    if (c.nb < 1 || c.nst < 2 || c.max_ages < 3 || c.n_times < 2)
         return set_error(WINDS_E_DIMS, "bad march sizes");
     if ((int64_t)(c.nst - 1) * c.nb > kMaxLL || c.nst > kMaxNodesBlade)
         return set_error(WINDS_E_DIMS, "too many lifting-line points for the KJ solver");
-    if (!c.r_trail || !c.chord_trail || !c.twist_trail || !c.alpha || !c.cl || !c.cd)
-        return set_error(WINDS_E_ARG, "null geometry or polar");
+    if (c.n_foils < 1 || c.n_foils > kMaxFoils || c.n_alpha < 2 || c.n_alpha > kMaxAlpha || !c.foil_len)
+        return set_error(WINDS_E_DIMS, "airfoil tables: 1..64 tables of 2..1024 entries in all");
+    {
+        int64_t tot = 0;
+        for (int k = 0; k < c.n_foils; ++k) {
+            if (c.foil_len[k] < 2) return set_error(WINDS_E_DIMS, "an airfoil table has < 2 entries");
+            tot += c.foil_len[k];
+        }
+        if (tot != c.n_alpha) return set_error(WINDS_E_DIMS, "airfoil table lengths do not add up");
+    }
+    if (!c.r_node || !c.chord_node || !c.twist_node || !c.dr_node || !c.foil || !c.cl_init ||
+        !c.alpha || !c.cl || !c.cd)
+        return set_error(WINDS_E_ARG, "null blade or airfoil array");
+    if (!c.time || !c.wind || !c.tsr || !c.rot_speed || !c.azimuth || !c.pitch || !c.hub ||
+        !c.quarter || !c.trail || !c.bound || !c.bxn || !c.byn || !c.bzn || !c.vblade)
+        return set_error(WINDS_E_ARG, "null kinematic table");
+    for (int i = 0; i < c.nst - 1; ++i)
+        if (c.foil[i] < 1 || c.foil[i] > c.n_foils)
+            return set_error(WINDS_E_ARG, "blade.NFoil names a table that does not exist");
     if (c.integrator < WINDS_INTEG_PCC || c.integrator > WINDS_INTEG_AB4)
         return set_error(WINDS_E_MODE, "unknown integrator");
-    if (!(c.dt > 0.0) || !(c.tsr > 0.0) || !(c.R > 0.0)) return set_error(WINDS_E_ARG, "bad dt/tsr/R");
+    if (!(c.dt > 0.0) || !(c.tip_radius > 0.0)) return set_error(WINDS_E_ARG, "bad dt/TipRad");
     winds_march *m = new winds_march();
     m->cfg = c;
     int rc = winds::opts_to_cfg(&c.wake, m->wake);
@@ -1288,26 +1331,39 @@ int winds_march_create(const winds_march_cfg *cfg, winds_march **out) {
     }
     // lifting-line sums have few targets: FP64 splits the segment range
     if (m->ll.precision == WINDS_PREC_FP64 && m->ll.split <= 1) m->ll.split = 256;
-    const int nst = c.nst, ns = c.nst - 1, nb = c.nb, na = c.n_alpha;
-    // geometry on the device: r,chord,twist (nst each), node r,chord,twist,dr (ns each), polar
-    std::vector<double> h(3 * nst + 4 * ns + 3 * na);
+    const int nst = c.nst, ns = c.nst - 1, nb = c.nb, na = c.n_alpha, nt = c.n_times;
+    const int nf = c.n_foils;
+    m->h_time.assign(c.time, c.time + nt);
+    m->h_wind.assign(c.wind, c.wind + 3 * (size_t)nt);
+    m->h_tsr.assign(c.tsr, c.tsr + nt);
+    m->h_azi.assign(c.azimuth, c.azimuth + nt);
+    m->h_hub.assign(c.hub, c.hub + 3 * (size_t)nt);
+    // device copy: doubles first, then the integer arrays
+    const size_t n_st = (size_t)nst * 3 * nt * nb, n_ns = (size_t)ns * 3 * nt * nb;
+    const size_t nd = 5 * (size_t)ns + 3 * (size_t)na + (size_t)nt * nb + nt + 2 * n_st +
+                      5 * n_ns + (c.ll_extra ? n_ns : 0);
+    const size_t ni = (size_t)ns + 2 * (size_t)nf;
+    std::vector<double> h(nd);
+    std::vector<int> hi(ni);
     double *q = h.data();
-    std::copy(c.r_trail, c.r_trail + nst, q);
-    std::copy(c.chord_trail, c.chord_trail + nst, q + nst);
-    std::copy(c.twist_trail, c.twist_trail + nst, q + 2 * nst);
-    double *rn = q + 3 * nst, *cn = rn + ns, *tn = cn + ns, *dr = tn + ns;
-    for (int i = 0; i < ns; ++i) {
-        // NRELrotor.m:48-51: RNodes = midpoints, linear interpolation, |diff|
-        rn[i] = c.r_trail[i] + (c.r_trail[i + 1] - c.r_trail[i]) / 2;
-        const double w = (rn[i] - c.r_trail[i]) / (c.r_trail[i + 1] - c.r_trail[i]);
-        cn[i] = c.chord_trail[i] + w * (c.chord_trail[i + 1] - c.chord_trail[i]);
-        tn[i] = c.twist_trail[i] + w * (c.twist_trail[i + 1] - c.twist_trail[i]);
-        dr[i] = fabs(c.r_trail[i + 1] - c.r_trail[i]);
+    auto put = [&](const double *src, size_t n) {
+        double *at = q;
+        std::copy(src, src + n, q);
+        q += n;
+        return at - h.data();
+    };
+    const size_t o_rn = put(c.r_node, ns), o_cn = put(c.chord_node, ns), o_tn = put(c.twist_node, ns),
+                 o_dr = put(c.dr_node, ns), o_cl0 = put(c.cl_init, ns), o_al = put(c.alpha, na),
+                 o_cl = put(c.cl, na), o_cd = put(c.cd, na), o_pi = put(c.pitch, (size_t)nt * nb),
+                 o_rs = put(c.rot_speed, nt), o_qu = put(c.quarter, n_st),
+                 o_tr = put(c.trail, n_st), o_bo = put(c.bound, n_ns), o_bx = put(c.bxn, n_ns),
+                 o_by = put(c.byn, n_ns), o_bz = put(c.bzn, n_ns), o_vb = put(c.vblade, n_ns);
+    const size_t o_ex = c.ll_extra ? put(c.ll_extra, n_ns) : 0;
+    for (int i = 0; i < ns; ++i) hi[i] = c.foil[i] - 1;
+    for (int k = 0, off = 0; k < nf; off += c.foil_len[k], ++k) {
+        hi[ns + k] = off;
+        hi[ns + nf + k] = c.foil_len[k];
     }
-    double *al = dr + ns;
-    std::copy(c.alpha, c.alpha + na, al);
-    std::copy(c.cl, c.cl + na, al + na);
-    std::copy(c.cd, c.cd + na, al + 2 * na);
     auto fail = [&](int code) {
         winds_march_destroy(m);
         return code;
@@ -1319,33 +1375,39 @@ int winds_march_create(const winds_march_cfg *cfg, winds_march **out) {
             return fail(set_error(WINDS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_))); \
     } while (0)
     WMC(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
-    WMC(m->geom.alloc(h.size() * sizeof(double)));
-    WMC(hipMemcpy(m->geom.p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    const size_t ibytes = ((ni * sizeof(int) + 255) / 256) * 256;
+    WMC(m->geom.alloc(nd * sizeof(double) + ibytes));
+    WMC(hipMemcpy(m->geom.p, h.data(), nd * sizeof(double), hipMemcpyHostToDevice));
+    int *GI = (int *)((char *)m->geom.p + nd * sizeof(double));
+    WMC(hipMemcpy(GI, hi.data(), ni * sizeof(int), hipMemcpyHostToDevice));
     const double *G = m->geom.d();
-    m->g.r_trail = G;
-    m->g.chord_trail = G + nst;
-    m->g.twist_trail = G + 2 * nst;
-    m->g.r_node = G + 3 * nst;
-    m->g.chord_node = m->g.r_node + ns;
-    m->g.twist_node = m->g.chord_node + ns;
-    m->g.dr_node = m->g.twist_node + ns;
-    m->g.alpha = m->g.dr_node + ns;
-    m->g.cl = m->g.alpha + na;
-    m->g.cd = m->g.cl + na;
+    m->g.r_node = G + o_rn;
+    m->g.chord_node = G + o_cn;
+    m->g.twist_node = G + o_tn;
+    m->g.dr_node = G + o_dr;
+    m->cl0 = G + o_cl0;
+    m->g.alpha = G + o_al;
+    m->g.cl = G + o_cl;
+    m->g.cd = G + o_cd;
+    m->g.pitch = G + o_pi;
+    m->g.rot_speed = G + o_rs;
+    m->g.quarter = G + o_qu;
+    m->g.trail = G + o_tr;
+    m->g.bound = G + o_bo;
+    m->g.bxn = G + o_bx;
+    m->g.byn = G + o_by;
+    m->g.bzn = G + o_bz;
+    m->g.vblade = G + o_vb;
+    m->g.ll_extra = c.ll_extra ? G + o_ex : nullptr;
+    m->g.foil = GI;
+    m->g.foil_off = GI + ns;
+    m->g.foil_len = GI + ns + nf;
     m->g.nb = nb;
     m->g.nst = nst;
     m->g.ns = ns;
+    m->g.nt = nt;
+    m->g.n_foils = nf;
     m->g.n_alpha = na;
-    m->g.omega = c.omega;
-    m->g.pitch_deg = c.pitch_deg;
-    m->g.ux = c.uinf[0];
-    m->g.uy = c.uinf[1];
-    m->g.uz = c.uinf[2];
-    const double umag = sqrt(c.uinf[0] * c.uinf[0] + c.uinf[1] * c.uinf[1] + c.uinf[2] * c.uinf[2]);
-    m->umag = umag;
-    // initials.m:134-135
-    const double T0 = 2 * kPi * c.R / (12 * c.tsr * umag);
-    m->r0 = sqrt(4 * kAlphaRL * kNu * kDelta * T0);
     const int AM = c.max_ages;
     const size_t NB = (size_t)3 * nst * AM * nb * sizeof(double);
     const size_t SS = (size_t)ns * AM * nb * sizeof(double);
@@ -1377,7 +1439,7 @@ int winds_march_create(const winds_march_cfg *cfg, winds_march **out) {
     WMC(m->frozen.alloc((size_t)3 * NLL * sizeof(double)));
     WMC(m->K.alloc((size_t)3 * NLL * NCH * sizeof(double)));
     WMC(m->kjout.alloc((size_t)3 * NLL * sizeof(double)));
-    WMC(m->perf.alloc((size_t)5 * c.max_steps * sizeof(double)));
+    WMC(m->perf.alloc((size_t)5 * nt * sizeof(double)));
     WMC(m->flag.alloc(256));
     WMC(hipMemsetAsync(m->flag.p, 0, 256, m->st));
     WMC(hipMemsetAsync(m->gt1.p, 0, ST, m->st));
@@ -1458,7 +1520,8 @@ int winds_march_set_state(winds_march *m, int32_t p, int32_t ages, const double 
                           const double *gshed, const double *gtrail) {
     winds::clear_error();
     if (!m || !nodes || !gshed || !gtrail) return set_error(WINDS_E_ARG, "null argument");
-    if (ages < 3 || ages > m->cfg.max_ages || p < 4) return set_error(WINDS_E_DIMS, "bad ages/p");
+    if (ages < 3 || ages > m->cfg.max_ages || p < 4 || p >= m->cfg.n_times)
+        return set_error(WINDS_E_DIMS, "bad ages/p");
     WM_HIP(hipStreamSynchronize(m->st));
     const int nst = m->g.nst, ns = m->g.ns, nb = m->g.nb, A = ages;
     const int64_t N = nodes_of(m, A);
@@ -1484,7 +1547,7 @@ int winds_march_set_state(winds_march *m, int32_t p, int32_t ages, const double 
     // lengths of time p-1: this wake's own lengths (zero strain on the first step)
     const int64_t ns_ = segs_of(m, A);
     k_vcore<<<blocks(ns_), 256, 0, m->st>>>(nst, nb, A, m->nodes[m->cur].d(), m->gs[m->gprev].d(),
-                                            m->gt[m->gprev].d(), m->r0, 0.0, 0, m->ls[0].d(),
+                                            m->gt[m->gprev].d(), r0_of(m, p), 0.0, 0, m->ls[0].d(),
                                             m->lt[0].d(), A, m->ls[m->lcur].d(), m->lt[m->lcur].d(),
                                             m->rs.d(), m->rt.d());
     WM_HIP(hipGetLastError());

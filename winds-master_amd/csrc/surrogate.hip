@@ -41,8 +41,8 @@ __global__ __launch_bounds__(kThreads) void k_desc(Geo g, const double *__restri
         const int64_t t0 = (int64_t)a * nst + (int64_t)b * nst * (A - 1);
         acc[0] += gt[t0 + nst - 1];
         acc[1] += gt[t0];
-        const double y = X[N + j], z = X[2 * N + j];
-        acc[2] += X[j];
+        const double y = X[N + j] - g.hub[1], z = X[2 * N + j] - g.hub[2];
+        acc[2] += X[j] - g.hub[0];
         acc[3] += sqrt(y * y + z * z);
         acc[4] += y;
     }
@@ -77,7 +77,7 @@ __global__ void k_features(Geo g, const float *__restrict__ desc, const double *
                            int64_t np, const float *__restrict__ norm, float *__restrict__ feat) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= np) return;
-    const double x = P[i], y = P[np + i], z = P[2 * np + i];
+    const double x = P[i] - g.hub[0], y = P[np + i] - g.hub[1], z = P[2 * np + i] - g.hub[2];
     const double c = cos(g.psi), s = sin(g.psi);
     float f[4];
     f[0] = (float)(x / g.R);

@@ -19,10 +19,12 @@ constexpr int kNOut = WINDS_SURR_NOUT;
 struct Geo {
     int nb, nst, ages, split, max_ages;
     double R, U, psi;
+    double hub[3];  // rotor centre: positions are taken relative to it
 };
 
 inline Geo from_abi(const winds_surr_geom &g) {
-    return Geo{g.nb, g.nst, g.ages, g.split, g.max_ages, g.R, g.U, g.psi};
+    return Geo{g.nb, g.nst, g.ages, g.split, g.max_ages, g.R, g.U, g.psi,
+               {g.hub[0], g.hub[1], g.hub[2]}};
 }
 
 int check_geo(const Geo &g);

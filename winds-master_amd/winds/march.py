@@ -12,15 +12,25 @@ INTEG = {"pcc": 0, "fe": 1, "ab4": 2}
 
 class MarchCfg(C.Structure):
     _fields_ = [
-        ("nb", C.c_int32), ("nst", C.c_int32), ("max_ages", C.c_int32), ("n_alpha", C.c_int32),
-        ("r_trail", C.c_void_p), ("chord_trail", C.c_void_p), ("twist_trail", C.c_void_p),
-        ("alpha", C.c_void_p), ("cl", C.c_void_p), ("cd", C.c_void_p),
-        ("R", C.c_double), ("tsr", C.c_double), ("omega", C.c_double), ("dt", C.c_double),
-        ("pitch_deg", C.c_double), ("uinf", C.c_double * 3), ("relax", C.c_double),
-        ("tol", C.c_double), ("maxiter", C.c_int32), ("integrator", C.c_int32),
-        ("max_steps", C.c_int32), ("reserved", C.c_int32),
+        ("nb", C.c_int32), ("nst", C.c_int32), ("max_ages", C.c_int32), ("n_times", C.c_int32),
+        ("r_node", C.c_void_p), ("chord_node", C.c_void_p), ("twist_node", C.c_void_p),
+        ("dr_node", C.c_void_p), ("foil", C.c_void_p), ("cl_init", C.c_void_p),
+        ("tip_radius", C.c_double), ("n_foils", C.c_int32), ("n_alpha", C.c_int32),
+        ("foil_len", C.c_void_p), ("alpha", C.c_void_p), ("cl", C.c_void_p), ("cd", C.c_void_p),
+        ("time", C.c_void_p), ("wind", C.c_void_p), ("tsr", C.c_void_p),
+        ("rot_speed", C.c_void_p), ("azimuth", C.c_void_p), ("pitch", C.c_void_p),
+        ("hub", C.c_void_p), ("quarter", C.c_void_p), ("trail", C.c_void_p),
+        ("bound", C.c_void_p), ("bxn", C.c_void_p), ("byn", C.c_void_p), ("bzn", C.c_void_p),
+        ("vblade", C.c_void_p), ("ll_extra", C.c_void_p),
+        ("dt", C.c_double), ("relax", C.c_double), ("tol", C.c_double),
+        ("maxiter", C.c_int32), ("integrator", C.c_int32),
         ("wake", _lib.BsOpts), ("ll", _lib.BsOpts),
     ]
+
+
+_TABLE_FIELDS = ("r_node", "chord_node", "twist_node", "dr_node", "foil", "cl_init", "foil_len",
+                 "alpha", "cl", "cd", "time", "wind", "tsr", "rot_speed", "azimuth", "pitch",
+                 "hub", "quarter", "trail", "bound", "bxn", "byn", "bzn", "vblade", "ll_extra")
 
 
 class MarchSurr(C.Structure):
@@ -68,30 +78,34 @@ def _dp(a):
 
 
 class March:
-    """Free-wake march of a rigid rotor (winds.rotor.Rotor) on one GPU."""
+    """Free-wake march on one GPU, driven by the kinematic tables of a case
+    (winds.cases.tables: NREL 5-MW, a synthetic rotor, the elliptical wing,
+    or any motion the caller states)."""
 
-    def __init__(self, rotor, max_ages: int, wake: _lib.BsOpts, ll: _lib.BsOpts | None = None,
+    def __init__(self, tables, max_ages: int, wake: _lib.BsOpts, ll: _lib.BsOpts | None = None,
                  integrator: str = "pcc", relax: float = 0.25, tol: float = 1e-8,
-                 maxiter: int = 30, max_steps: int = 20000):
+                 maxiter: int = 30):
         import torch  # the engine binds to torch's HIP runtime when torch is present
         torch.cuda.init()
         self.L = _bind(_lib.lib())
-        self.rotor = rotor
-        keep = [np.ascontiguousarray(x, dtype=np.float64) for x in
-                (rotor.r_trail, rotor.chord_trail, rotor.twist_trail, rotor.alpha, rotor.cl,
-                 rotor.cd)]
-        self._keep = keep
+        self.tables = tables
         c = MarchCfg()
-        c.nb, c.nst, c.max_ages, c.n_alpha = rotor.nb, rotor.nst, max_ages, rotor.alpha.size
-        c.r_trail, c.chord_trail, c.twist_trail, c.alpha, c.cl, c.cd = [
-            x.ctypes.data for x in keep]
-        c.R, c.tsr, c.omega, c.dt = rotor.R, rotor.tsr, rotor.omega, rotor.dt
-        c.pitch_deg = rotor.pitch_deg
-        for i in range(3):
-            c.uinf[i] = float(rotor.uinf[i])
-        c.relax, c.tol, c.maxiter = relax, tol, maxiter
+        keep = {}
+        for name in _TABLE_FIELDS:
+            a = getattr(tables, name)
+            if a is None:
+                setattr(c, name, None)
+                continue
+            dt = np.int32 if name in ("foil", "foil_len") else np.float64
+            a = np.asfortranarray(a, dtype=dt)
+            keep[name] = a
+            setattr(c, name, a.ctypes.data)
+        self._keep = keep
+        c.nb, c.nst, c.max_ages, c.n_times = tables.nb, tables.nst, max_ages, tables.n_times
+        c.tip_radius = tables.tip_radius
+        c.n_foils, c.n_alpha = int(tables.foil_len.size), int(tables.alpha.size)
+        c.dt, c.relax, c.tol, c.maxiter = tables.dt, relax, tol, maxiter
         c.integrator = INTEG[integrator]
-        c.max_steps = max_steps
         c.wake = wake
         c.ll = ll if ll is not None else _lib.opts(
             core=wake.core, vat_n=wake.vat_n, semantics=wake.semantics,
@@ -124,7 +138,7 @@ class March:
     def state(self):
         """(nodes [nst,3,A,nb] of the next time index, shed [ns,1,A',nb] and
         trailing [nst,1,A'-1,nb] circulation of the last solved time index)."""
-        r = self.rotor
+        r = self.tables
         inf = self.info()
         A = inf["ages"]
         nodes = np.zeros((r.nst, 3, A, r.nb), order="F")
@@ -135,7 +149,7 @@ class March:
         return nodes, gs, gt
 
     def state_circulation(self, ages_solved: int):
-        r = self.rotor
+        r = self.tables
         _, gs, gt = self.state()
         A = ages_solved
         gs = gs[: r.ns * A * r.nb].reshape((r.ns, 1, A, r.nb), order="F")
@@ -159,7 +173,7 @@ class March:
         return out
 
     def kj(self):
-        r = self.rotor
+        r = self.tables
         cl, cd, aoa = (np.zeros(r.ns * r.nb) for _ in range(3))
         _lib.check(self.L.winds_march_get_kj(self._m, _dp(cl), _dp(cd), _dp(aoa)))
         f = lambda a: a.reshape((r.ns, r.nb), order="F")

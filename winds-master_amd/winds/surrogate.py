@@ -28,7 +28,8 @@ NOUT = 3
 class SurrGeom(C.Structure):
     _fields_ = [("nb", C.c_int32), ("nst", C.c_int32), ("ages", C.c_int32),
                 ("split", C.c_int32), ("max_ages", C.c_int32), ("reserved", C.c_int32),
-                ("R", C.c_double), ("U", C.c_double), ("psi", C.c_double)]
+                ("R", C.c_double), ("U", C.c_double), ("psi", C.c_double),
+                ("hub", C.c_double * 3)]
 
 
 def _bind(L):
@@ -83,10 +84,12 @@ def query_points(w: gen.RotorWake, target_ages: int) -> np.ndarray:
     return np.ascontiguousarray(np.concatenate([near, ll], axis=1))
 
 
-def geom(nb, nst, ages, split, max_ages, R, U, psi) -> SurrGeom:
+def geom(nb, nst, ages, split, max_ages, R, U, psi, hub=(0.0, 0.0, 0.0)) -> SurrGeom:
     g = SurrGeom()
     g.nb, g.nst, g.ages, g.split, g.max_ages = nb, nst, ages, split, max_ages
     g.R, g.U, g.psi = R, U, psi
+    for i in range(3):
+        g.hub[i] = float(hub[i])
     return g
 
 
@@ -181,12 +184,19 @@ def build_training_set(n_samples: int, opts, seed: int = 7, split: int = 144,
     return TrainingSet(F, Y, nw, inter, time.perf_counter() - t0)
 
 
+def march_psi(tables, p: int) -> float:
+    """Angle of the frame turning with the rotor at time index p: blade 0's
+    azimuth (fastout.Azimuth) plus 90 degrees, so that psi = 0 puts blade 0
+    along +y (the march's own convention, psi_of in march.hip)."""
+    return math.radians(float(tables.azimuth[p - 1])) + 0.5 * math.pi
+
+
 def march_wake(m) -> tuple:
     """The wake of a winds.march.March at its next time index p as a
     RotorWake (circulation of time p: time p-1's values moved one age older,
-    age 0 repeated; cores zero, which the cut-off kernels ignore) and the
-    blade-0 azimuth Omega (p-1) dt."""
-    r = m.rotor
+    age 0 repeated; cores zero, which the cut-off kernels ignore), the
+    turning-frame angle and the rotor centre at p."""
+    r = m.tables
     inf = m.info()
     p, Ag = inf["p"], inf["ages_solved"]
     nodes, gs, gt = m.state()
@@ -199,11 +209,12 @@ def march_wake(m) -> tuple:
     w = gen.RotorWake(domain=np.asfortranarray(nodes), gamma_shed=np.asfortranarray(gs_p),
                       gamma_trail=np.asfortranarray(gt_p),
                       rc_shed=np.zeros_like(gs_p, order="F"), rc_trail=np.zeros_like(gt_p, order="F"),
-                      dt=r.dt, omega=r.omega)
-    return w, r.omega * (p - 1) * r.dt
+                      dt=r.dt, omega=float(r.rot_speed[p - 1]) * math.pi / 30)
+    return w, march_psi(r, p), np.asarray(r.hub)[p - 1]
 
 
-def wake_samples(w, psi, opts, split, target_ages, max_ages, R, U, ev=None):
+def wake_samples(w, psi, opts, split, target_ages, max_ages, R, U, ev=None,
+                 hub=(0.0, 0.0, 0.0)):
     """(raw inputs, raw targets, interactions) of one wake: the far segments'
     direct sum at the near-wake nodes and lifting-line points."""
     import torch
@@ -217,7 +228,7 @@ def wake_samples(w, psi, opts, split, target_ages, max_ages, R, U, ev=None):
     Pd = torch.from_numpy(P).cuda()
     Sd = torch.from_numpy(np.ascontiguousarray(segs)).cuda()
     u = ev.eval(Pd, Sd, opts)
-    g = geom(w.nb, w.nst, w.ages, split, max_ages, R, U, psi)
+    g = geom(w.nb, w.nst, w.ages, split, max_ages, R, U, psi, hub)
     return features(g, Xd, gtd, Pd), labels(g, u), P.shape[1] * segs.shape[0]
 
 
@@ -229,8 +240,7 @@ def march_training_set(m, opts, steps: int, every: int, split: int, target_ages:
     import torch
     from .device import Evaluator
     ev = Evaluator(0)
-    r = m.rotor
-    U = float(np.linalg.norm(r.uinf))
+    r = m.tables
     max_ages = m.cfg.max_ages
     Fs, Ys, inter, snaps = [], [], 0, 0
     t0 = time.perf_counter()
@@ -242,8 +252,10 @@ def march_training_set(m, opts, steps: int, every: int, split: int, target_ages:
         done += k
         if m.info()["ages"] <= split + 1:
             continue
-        w, psi = march_wake(m)
-        F, Y, n = wake_samples(w, psi, opts, split, target_ages, max_ages, r.R, U, ev)
+        w, psi, hub = march_wake(m)
+        U = float(np.linalg.norm(np.asarray(r.wind)[m.info()["p"] - 1]))
+        F, Y, n = wake_samples(w, psi, opts, split, target_ages, max_ages, r.tip_radius, U, ev,
+                               hub=hub)
         Fs.append(F)
         Ys.append(Y)
         inter += n
