@@ -41,6 +41,13 @@ def test_every_declared_symbol_is_exported():
     assert len(sur) == 3
     for n in sur:
         assert hasattr(L, n), n
+    gnd = header_functions("winds_ground.h")
+    assert len(gnd) == 6
+    for n in gnd:
+        assert hasattr(L, n), n
+    # every header in include/ is covered above
+    assert {p.name for p in (ROOT / "include").glob("*.h")} == {
+        "winds_bs.h", "winds_march.h", "winds_mlp.h", "winds_surrogate.h", "winds_ground.h"}
 
 
 def test_version_and_defaults():

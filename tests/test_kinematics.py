@@ -189,3 +189,31 @@ def test_nrel_blade_nodes():
     assert np.array_equal(b.NFoil, [1, 1, 2, 3, 4, 4, 5, 6, 6, 7, 7, 8, 8, 8, 8, 8, 8, 8])
     assert np.allclose(b.DRNodes, np.diff(cases.NREL_RTRAIL))
     assert np.allclose(b.Chord[0], 3.542) and np.allclose(b.AeroTwst[0], -13.308)
+
+
+def test_tower_shadow_is_the_flow_around_a_cylinder():
+    """tower_shadow.m: the doublet flow U e_x + V around the tower has no
+    normal component on the cylinder surface, decays as 1/r^2, is zero above
+    the tower and inside it; a downwind rotor's wake region gets -0.3 U."""
+    from types import SimpleNamespace
+    from winds import tower
+    tur = SimpleNamespace(TowerHt=90.0, TowerBaseDiam=6.0, TowerTopDiam=4.0)
+    U = 11.0
+    th = np.linspace(-np.pi, np.pi, 37)[:-1]
+    z = 45.0
+    R = (6.0 + (4.0 - 6.0) * z / 90.0) / 2
+    for rr in (R * (1 + 1e-12), 2 * R, 4 * R):
+        P = np.stack([rr * np.cos(th), rr * np.sin(th), np.full(th.size, z)], 1)[:, :, None]
+        V = tower.tower_shadow(P, U, 1, tur)[:, :, 0, 0]
+        if rr == R * (1 + 1e-12):
+            vn = (U + V[:, 0]) * np.cos(th) + V[:, 1] * np.sin(th)
+            assert np.max(np.abs(vn)) <= 1e-9 * U
+        mag = np.hypot(V[:, 0], V[:, 1])
+        assert np.allclose(mag, U * (R / rr) ** 2, rtol=1e-12)
+        assert np.all(V[:, 2] == 0)
+    P = np.array([[3.0, 0.0, 95.0], [0.5, 0.5, 10.0], [-20.0, 1.0, 10.0]])[:, :, None]
+    V = tower.tower_shadow(P, U, 1, tur)
+    assert np.all(V[0] == 0) and np.all(V[1] == 0) and np.any(V[2] != 0)
+    Pd = np.array([[10.0, 2.0, 30.0], [-10.0, 2.0, 30.0]])[:, :, None]
+    Vd = tower.tower_shadow(Pd, U, 2, tur)[:, :, 0, 0]
+    assert Vd[0, 0] == -0.3 * U and Vd[0, 1] == 0 and Vd[1, 0] != -0.3 * U

@@ -139,3 +139,18 @@ def test_tables_end_is_an_error(need_gpu):
     m.step(8)                     # time indices 2..9 need tables up to 10
     with pytest.raises(winds.WindsError, match="tables end"):
         m.step(1)
+
+
+def test_nrel_5mw_with_tower_shadow_matches_oracle(need_gpu):
+    """user.towerflag 'true': the tower's potential-flow velocity at the
+    bound points (tower_shadow.m) joins the wake's induced velocity in the
+    Kutta-Joukowski solve (KuttaJoukowski.m:232-238); the rotor sits 5 m
+    downwind of the tower (user.toweroverhang = 5), downwind model."""
+    tb = cases.tables(cases.nrel_rotor(t=(0.0, 2.0, 10.0)), tower=True, up_down=2)
+    assert np.any(tb.ll_extra != 0)
+    m, om = run_pair(tb, 12, 16, "mex", 1e-4, "pcc")
+    compare(m, om, "NREL 5-MW tower shadow")
+    # the tower changes the loads
+    plain = cases.tables(cases.nrel_rotor(t=(0.0, 2.0, 10.0)))
+    m2, _ = run_pair(plain, 12, 16, "mex", 1e-4, "pcc")
+    assert np.max(np.abs(m.perf()[:, 1] - m2.perf()[:, 1])) > 1e-4 * np.max(np.abs(m2.perf()[:, 1]))

@@ -26,6 +26,7 @@ SOURCES = [
     "march.hip",
     "mlp.hip",
     "surrogate.hip",
+    "ground.hip",
 ]
 
 ARCH = "gfx950"

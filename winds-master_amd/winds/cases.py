@@ -230,14 +230,20 @@ class Tables(SimpleNamespace):
     kinematic tables in the reference's layouts (include/winds_march.h)."""
 
 
-def tables(case, cl_init=None, ll_extra=None) -> Tables:
+def tables(case, cl_init=None, ll_extra=None, tower: bool = False, up_down: int = 1) -> Tables:
     """kinematics.m + velocity.m on the case, and the initial lift
     coefficient: BEM.m's at the first time index (initials.m:116-119), or
     for the elliptical wing cl = 2 pi alpha / (1 + 2/AR) at the geometric
-    angle of attack (initials.m:111-113)."""
+    angle of attack (initials.m:111-113).  tower=True adds the tower's
+    induced velocity at the lifting line (tower_shadow.m, user.towerflag;
+    up_down = user.towerupdown: 1 upwind, 2 downwind rotor) to ll_extra."""
+    from . import tower as tw
     blade, turbine, platform, fo = case.blade, case.turbine, case.platform, case.fastout
     pos = kin.kinematics(blade, turbine, platform, fo)
     vel, pos = kin.velocity(pos, blade, turbine, case.wind, fo)
+    if tower:
+        tt = tw.tower_table(pos, case.wind, turbine, up_down)
+        ll_extra = tt if ll_extra is None else np.asfortranarray(ll_extra + tt)
     nb = int(turbine.NumBl)
     nt = fo.Time.size
     if cl_init is None:
