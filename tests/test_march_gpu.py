@@ -10,7 +10,8 @@ Kutta-Joukowski iteration and those it changes).  Cases: a small synthetic
 rotor in every integrator and core, the NREL 5-MW rotor (eight airfoil
 tables, shaft tilt, precone, tower height and overhang), the same rotor on a
 surging and pitching platform with yaw, and the 3 x 31-node, 5-degree
-geometry of the full-length benchmark march."""
+geometry of the full-length benchmark march, and the rotor read from a FAST
+file set."""
 import math
 from functools import lru_cache
 
@@ -154,3 +155,16 @@ def test_nrel_5mw_with_tower_shadow_matches_oracle(need_gpu):
     plain = cases.tables(cases.nrel_rotor(t=(0.0, 2.0, 10.0)))
     m2, _ = run_pair(plain, 12, 16, "mex", 1e-4, "pcc")
     assert np.max(np.abs(m.perf()[:, 1] - m2.perf()[:, 1])) > 1e-4 * np.max(np.abs(m2.perf()[:, 1]))
+
+
+def test_fast_file_set_matches_oracle(need_gpu, tmp_path):
+    """A FAST v7 file set (.fst, AeroDyn file, eight airfoil files, .out) of
+    the NREL 5-MW rotor, imported by winds.fast (input_import.m,
+    output_import.m, input_mod.m): 17 AeroDyn sections, the tip at half
+    chord, the azimuth unwrapped from the output's 0-360 channel."""
+    import fastfiles as FF
+    from winds import fast
+    tb = cases.tables(fast.fast_case(FF.nrel_file_set(tmp_path / "fast"), (0.0, 2.0, 10.0)))
+    assert tb.ns == 17 and tb.nst == 18
+    m, om = run_pair(tb, 12, 16, "mex", 1e-4, "pcc")
+    compare(m, om, "FAST file set")
