@@ -51,7 +51,10 @@ int winds_mlp_gradients(winds_mlp *m, const float *X, const float *T, int32_t B,
 
 /* One Adam step on the MSE loss of the batch (B <= max_batch) on stream
  * (NULL: the default stream).  loss (device FP32 scalar, may be NULL)
- * receives the batch loss before the update. */
+ * receives the batch loss before the update.  From the second of
+ * consecutive calls with the same X, T, B and stream, the step (about 30
+ * kernels) is recorded once into a HIP graph, on the network's own stream,
+ * and replayed on stream; X and T are read when each step runs. */
 int winds_mlp_train_step(winds_mlp *m, const float *X, const float *T, int32_t B, float lr,
                          float *loss, void *stream);
 

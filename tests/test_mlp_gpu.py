@@ -135,3 +135,36 @@ def test_batch_limits(need_gpu):
         net.forward(X)
     Y = net.forward(X[:0])
     assert Y.shape == (0, 3)
+
+
+def test_graph_replayed_steps_match_eager_steps(need_gpu):
+    """Repeated train_step calls on the same batch buffers replay a recorded
+    graph of the step from the second call on; a twin network fed the same
+    values from alternating buffers runs every step eagerly.  Between steps
+    the program does eager work of its own (a forward pass, torch kernels,
+    synchronisations); after every step both networks hold the same
+    parameters bit for bit, the losses agree (to the rounding of their
+    atomic sum), and the learning rate and Adam
+    scalars of each replay are the current ones (they differ every step)."""
+    import torch
+    a = make(hidden=256, layers=4, n_in=49, max_batch=1024)
+    b = make(hidden=256, layers=4, n_in=49, max_batch=1024)
+    assert np.array_equal(a.get_params(), b.get_params())
+    torch.manual_seed(3)
+    X = torch.randn(1000, 49, device="cuda")
+    T = torch.randn(1000, 3, device="cuda") * 0.3
+    X2, T2 = X.clone(), T.clone()
+    la = torch.zeros(1, device="cuda")
+    lb = torch.zeros(1, device="cuda")
+    for it in range(6):
+        lr = 1e-3 * (1.0 + 0.5 * it)
+        a.train_step(X, T, lr, la)                       # same buffers: graph from step 2
+        xb, tb = (X, T) if it % 2 else (X2, T2)
+        b.train_step(xb, tb, lr, lb)                     # alternating buffers: eager
+        _ = a.forward(X[:100])                           # eager work between replays
+        X.add_(0.01)
+        X2.add_(0.01)
+        torch.cuda.synchronize()
+        assert np.array_equal(a.get_params(), b.get_params()), it
+        # the loss is an atomic sum over workgroups: equal to rounding
+        assert abs(float(la.item()) - float(lb.item())) <= 1e-6 * float(lb.item()), it

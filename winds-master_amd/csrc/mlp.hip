@@ -1,12 +1,17 @@
 // mlp.hip -- BF16 MFMA multilayer perceptron for the far-wake surrogate.
 //
 // Every product is one NT GEMM, C[M][N] = A[M][K] * B[N][K]^T with BF16
-// operands (K contiguous in both) and FP32 accumulation in
-// v_mfma_f32_32x32x8bf16_1k: lane l feeds A row l%32 and B row l%32 at
-// k = 4*(l/32)..+3 and receives C[8*(v/4) + 4*(l/32) + v%4][l%32] in
-// accumulator v.  Keeping activations and deltas in both [batch][feature]
-// and [feature][batch] layouts lets the forward pass (X W^T), the input
-// gradient (dZ W) and the weight gradient (dZ^T X) all use that one kernel.
+// operands (K contiguous in both) and FP32 accumulation in gfx950's
+// v_mfma_f32_32x32x16_bf16: lane l feeds A row l%32 and B row l%32 with
+// eight consecutive k of each 16-k step (the same k for A and B, which is
+// all the sum over k needs) and receives C[8*(v/4) + 4*(l/32) + v%4][l%32] in
+// accumulator v.  A workgroup computes a 64 x 64 block of C from 64-wide K
+// slices staged in LDS (double buffered, one barrier per slice), each wave a
+// 32 x 32 block; the epilogue applies the layer's bias and SiLU (forward) or
+// SiLU's derivative (backward) and writes the BF16 activations or deltas in
+// both [batch][feature] and [feature][batch] layouts, so that the forward
+// pass (X W^T), the input gradient (dZ W) and the weight gradient (dZ^T X)
+// all use this one kernel.
 
 #include <hip/hip_runtime.h>
 
@@ -36,56 +41,148 @@ __device__ __forceinline__ bf16 to_bf16(float x) {
 __device__ __forceinline__ float from_bf16(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
 
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
 
-// C[M][N] = A[M][K] * B[N][K]^T; M, N multiples of 32, K a multiple of 32,
-// lda/ldb multiples of 32.  Workgroup: 4 waves, a 64x64 block of C, one
-// 32x32 block per wave.  Each 32-wide K block is fed as four MFMAs from two
-// 16-byte loads per lane and operand: lane l holds k in [16*(l/32), +16) of
-// the block, and MFMA q takes its elements 4q..4q+3 (A and B use the same
-// assignment of k to MFMA steps, which the sum over k does not see).  The
-// next block's loads are issued before the current block's MFMAs.
-__global__ __launch_bounds__(256) void gemm_nt(const bf16 *__restrict__ A, int lda,
-                                               const bf16 *__restrict__ B, int ldb,
-                                               float *__restrict__ C, int ldc, int M, int N,
-                                               int K) {
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int tr = blockIdx.y * 64 + (w >> 1) * 32;
-    const int tc = blockIdx.x * 64 + (w & 1) * 32;
-    if (tr >= M || tc >= N) return;
-    const bf16 *ap = A + (int64_t)(tr + (l & 31)) * lda + 16 * (l >> 5);
-    const bf16 *bp = B + (int64_t)(tc + (l & 31)) * ldb + 16 * (l >> 5);
-    f32x16 acc = {0};
-    bf16x8 a0 = *(const bf16x8 *)ap, a1 = *(const bf16x8 *)(ap + 8);
-    bf16x8 b0 = *(const bf16x8 *)bp, b1 = *(const bf16x8 *)(bp + 8);
-    for (int k = 0; k < K; k += 32) {
-        bf16x8 na0 = a0, na1 = a1, nb0 = b0, nb1 = b1;
-        if (k + 32 < K) {
-            na0 = *(const bf16x8 *)(ap + k + 32);
-            na1 = *(const bf16x8 *)(ap + k + 40);
-            nb0 = *(const bf16x8 *)(bp + k + 32);
-            nb1 = *(const bf16x8 *)(bp + k + 40);
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + expf(-z)); }
+
+// What the GEMM epilogue does with each element x of the product.
+enum Epi : int {
+    kEpiF32 = 0,       // C = x
+    kEpiBias = 1,      // C = x + bias[col]
+    kEpiBiasSilu = 2,  // z = x + bias[col]; C = z (if C); a = SiLU(z) -> Ab, AT
+    kEpiSiluBack = 3,  // d = x SiLU'(Zin); C = d; d -> Ab, AT
+};
+
+struct EpiArgs {
+    float *C;
+    int ldc;
+    const float *bias;
+    const float *Zin;
+    int ldz;
+    bf16 *Ab;   // [M][N] BF16 copy (row stride lda_b)
+    int lda_b;
+    bf16 *AT;   // [N][M] BF16 copy (row stride ldt), may be null
+    int ldt;
+};
+
+constexpr int kTile = 64;            // C block of a workgroup (rows and columns)
+constexpr int kSlice = 128;          // K per LDS slice
+constexpr int kLdsRow = kSlice + 8;  // padded LDS row: 272 bytes, staggers the banks
+constexpr int kMaxSplit = 16;        // K splits of the weight-gradient product
+
+// C[M][N] = A[M][K] * B[N][K]^T with the epilogue EPI.  K a multiple of 32;
+// lda, ldb multiples of 8; rows of A beyond M and of B beyond N are read as
+// zeros and their results dropped.  256 threads.  Split K (kEpiF32 only):
+// workgroup z sums k in [z kspan, (z+1) kspan) into C + z slab.
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_bf16(const bf16 *__restrict__ A, int lda,
+                                                 const bf16 *__restrict__ B, int ldb, int M,
+                                                 int N, int K, int kspan, int64_t slab,
+                                                 EpiArgs e) {
+    __shared__ __attribute__((aligned(16))) bf16 sA[2][kTile][kLdsRow];
+    __shared__ __attribute__((aligned(16))) bf16 sB[2][kTile][kLdsRow];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int row0 = blockIdx.y * kTile, col0 = blockIdx.x * kTile;
+    const int kb = blockIdx.z * kspan, ke = min(K, kb + kspan);
+    // loader: a 64 x 128 slice of each operand is 1024 pieces of 8 BF16, four
+    // per thread (piece idx: row idx / 16, k 8 (idx % 16)); K and the split
+    // bounds are multiples of 32, so a piece is wholly inside or outside
+    bf16x8 ra[4], rb[4];
+    const bf16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int idx = t + 256 * i, r = idx >> 4, k = k0 + (idx & 15) * 8;
+            ra[i] = (k < ke && row0 + r < M) ? *(const bf16x8 *)(A + (int64_t)(row0 + r) * lda + k)
+                                             : zero;
+            rb[i] = (k < ke && col0 + r < N) ? *(const bf16x8 *)(B + (int64_t)(col0 + r) * ldb + k)
+                                             : zero;
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(__builtin_shufflevector(a0, a0, 0, 1, 2, 3),
-                                                       __builtin_shufflevector(b0, b0, 0, 1, 2, 3),
-                                                       acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(__builtin_shufflevector(a0, a0, 4, 5, 6, 7),
-                                                       __builtin_shufflevector(b0, b0, 4, 5, 6, 7),
-                                                       acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(__builtin_shufflevector(a1, a1, 0, 1, 2, 3),
-                                                       __builtin_shufflevector(b1, b1, 0, 1, 2, 3),
-                                                       acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(__builtin_shufflevector(a1, a1, 4, 5, 6, 7),
-                                                       __builtin_shufflevector(b1, b1, 4, 5, 6, 7),
-                                                       acc, 0, 0, 0);
-        a0 = na0;
-        a1 = na1;
-        b0 = nb0;
-        b1 = nb1;
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int idx = t + 256 * i, r = idx >> 4, c = (idx & 15) * 8;
+            *(bf16x8 *)&sA[buf][r][c] = ra[i];
+            *(bf16x8 *)&sB[buf][r][c] = rb[i];
+        }
+    };
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    f32x16 acc = {0};
+    load(kb);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kb; k0 < ke; k0 += kSlice) {
+        const bool more = k0 + kSlice < ke;
+        if (more) load(k0 + kSlice);
+#pragma unroll
+        for (int s = 0; s < kSlice / 16; ++s) {
+            const bf16x8 a = *(const bf16x8 *)&sA[buf][wr + (l & 31)][16 * s + 8 * (l >> 5)];
+            const bf16x8 b = *(const bf16x8 *)&sB[buf][wc + (l & 31)][16 * s + 8 * (l >> 5)];
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8v, a),
+                                                          __builtin_bit_cast(bf16x8v, b), acc, 0,
+                                                          0, 0);
+        }
+        // the other buffer was last read before the previous barrier
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
     }
-    const int col = tc + (l & 31);
-    for (int v = 0; v < 16; ++v) {
-        const int row = tr + 8 * (v >> 2) + 4 * (l >> 5) + (v & 3);
-        C[(int64_t)row * ldc + col] = acc[v];
+    if constexpr (EPI == kEpiF32) e.C += blockIdx.z * slab;
+    const int col = col0 + wc + (l & 31);
+    if constexpr (EPI == kEpiF32 || EPI == kEpiBias) {
+        if (col >= N) return;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int row = row0 + wr + 8 * (v >> 2) + 4 * (l >> 5) + (v & 3);
+            if (row >= M) continue;
+            const float x = EPI == kEpiBias ? acc[v] + e.bias[col] : acc[v];
+            e.C[(int64_t)row * e.ldc + col] = x;
+        }
+    } else {
+        // FP32 results straight out; the BF16 ones staged as the 64 x 64
+        // tile in LDS (the slices are no longer read) and written to both
+        // copies with 16-byte stores
+        bf16(*tile)[kTile + 8] = reinterpret_cast<bf16(*)[kTile + 8]>(&sA[0][0][0]);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int rl = wr + 8 * (v >> 2) + 4 * (l >> 5) + (v & 3), row = row0 + rl;
+            bf16 h = 0;
+            if (row < M && col < N) {
+                const int64_t o = (int64_t)row * e.ldc + col;
+                if constexpr (EPI == kEpiBiasSilu) {
+                    const float z = acc[v] + e.bias[col];
+                    if (e.C) e.C[o] = z;
+                    h = to_bf16(z * sigm(z));
+                } else {
+                    const float z = e.Zin[(int64_t)row * e.ldz + col];
+                    const float sg = sigm(z);
+                    const float d = acc[v] * (sg * (1.0f + z * (1.0f - sg)));
+                    e.C[o] = d;
+                    h = to_bf16(d);
+                }
+            }
+            tile[rl][wc + (l & 31)] = h;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int idx = t + 256 * i;
+            // [M][N] copy: row idx / 8, columns 8 (idx % 8) .. +7
+            const int r = idx >> 3, c8 = (idx & 7) * 8;
+            if (row0 + r < M && col0 + c8 < N)
+                *(bf16x8 *)(e.Ab + (int64_t)(row0 + r) * e.lda_b + col0 + c8) =
+                    *(const bf16x8 *)&tile[r][c8];
+            // [N][M] copy: column idx / 8, rows 8 (idx % 8) .. +7
+            const int c = idx >> 3, r8 = (idx & 7) * 8;
+            if (e.AT && col0 + c < N && row0 + r8 < M) {
+                bf16x8 q;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) q[j] = (short)tile[r8 + j][c];
+                *(bf16x8 *)(e.AT + (int64_t)(col0 + c) * e.ldt + row0 + r8) = q;
+            }
+        }
     }
 }
 
@@ -114,24 +211,6 @@ __global__ void pack_input(const float *__restrict__ X, int B, int n, int Bp, in
     const bf16 h = to_bf16(x);
     Xb[i] = h;
     if (XT) XT[(int64_t)k * Bp + b] = h;
-}
-
-__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + expf(-z)); }
-
-// Z += bias; hidden: A = SiLU(Z) in both layouts
-__global__ void bias_act(float *__restrict__ Z, const float *__restrict__ bias, int Bp, int N,
-                         int silu, bf16 *__restrict__ Ab, bf16 *__restrict__ AT) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)Bp * N) return;
-    int b, n;
-    row_col(i, N, b, n);
-    const float z = Z[i] + bias[n];
-    Z[i] = z;
-    if (silu) {
-        const bf16 h = to_bf16(z * sigm(z));
-        Ab[i] = h;
-        if (AT) AT[(int64_t)n * Bp + b] = h;
-    }
 }
 
 // dZ = 2 (Y - T) / (B n_out) on the real rows/columns, 0 elsewhere; loss sum
@@ -165,13 +244,12 @@ __global__ void mse_grad(const float *__restrict__ Y, const float *__restrict__ 
     }
 }
 
-// The same three conversions when the [column][row] copy is wanted too
-// (training), over 64 x 64 tiles through LDS so that both copies are
-// written with consecutive lanes on consecutive addresses (the per-element
-// kernels' transposed stores touched one 2-byte word per lane per line).
-// MODE 0: input rows X [B][n] (zero padded) -> [Bp][N] and [N][Bp];
-// MODE 1: Z += bias, A = SiLU(Z) -> [Bp][N] and [N][Bp];
-// MODE 2: delta dZ -> [Bp][N] and [N][Bp].  Bp is a multiple of 64.
+// BF16 copies in both layouts (training), over 64 x 64 tiles through LDS so
+// that both copies are written with consecutive lanes on consecutive
+// addresses (per-element transposed stores touch one 2-byte word per lane
+// per line).  MODE 0: input rows X [B][n] (zero padded) -> [Bp][N] and
+// [N][Bp]; MODE 2: the output layer's delta dZ -> [Bp][N] and [N][Bp].  Bp
+// is a multiple of 64.
 template <int MODE>
 __global__ __launch_bounds__(256) void tile_pack(float *__restrict__ Z, const float *__restrict__ X,
                                                  const float *__restrict__ bias, int B, int n,
@@ -191,10 +269,6 @@ __global__ __launch_bounds__(256) void tile_pack(float *__restrict__ Z, const fl
             float v;
             if constexpr (MODE == 0) {
                 v = (row < B && col < n) ? X[(int64_t)row * n + col] : 0.0f;
-            } else if constexpr (MODE == 1) {
-                const float z = Z[i] + bias[col];
-                Z[i] = z;
-                v = z * sigm(z);
             } else {
                 v = Z[i];
             }
@@ -212,42 +286,88 @@ __global__ __launch_bounds__(256) void tile_pack(float *__restrict__ Z, const fl
     }
 }
 
-// dZ = dA * SiLU'(Z)
-__global__ void silu_back(const float *__restrict__ dA, const float *__restrict__ Z, int64_t n,
-                          float *__restrict__ dZ) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float z = Z[i];
-    const float s = sigm(z);
-    dZ[i] = dA[i] * (s * (1.0f + z * (1.0f - s)));
-}
-
-// db[n] = sum over rows of dZ[b][n]
-__global__ void col_sum(const float *__restrict__ dZ, int Bp, int N, float *__restrict__ db) {
-    const int n = blockIdx.x;
+// Partial column sums of dZ [Bp][N] over blocks of 64 rows: workgroup
+// (x, z) sums rows 64z..64z+63 of columns 64x..64x+63 into db [z][N]; each
+// thread adds 16 rows in order, the four quarters are then added in order
+__global__ __launch_bounds__(256) void col_sum_part(const float *__restrict__ dZ, int Bp, int N,
+                                                    float *__restrict__ db) {
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63), q = threadIdx.x >> 6;
+    const int r0 = blockIdx.y * 64 + 16 * q;
     float s = 0.0f;
-    for (int b = threadIdx.x; b < Bp; b += blockDim.x) s += dZ[(int64_t)b * N + n];
-    __shared__ float red[256];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
+    if (c < N) {
+        float v[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = dZ[(int64_t)(r0 + j) * N + c];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) s += v[j];
     }
-    if (threadIdx.x == 0) db[n] = red[0];
+    __shared__ float red[4][64];
+    red[q][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (q == 0 && c < N) {
+        const int j = threadIdx.x & 63;
+        db[(int64_t)blockIdx.y * N + c] = ((red[0][j] + red[1][j]) + red[2][j]) + red[3][j];
+    }
 }
 
-__global__ void adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
-                     float *__restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
-                     float c1, float c2) {
+// Adam's per-step scalars {learning rate, 1 - b1^t, 1 - b2^t} into device
+// memory, so that a replayed graph of the step reads the current ones
+__global__ void set_hyper(float *__restrict__ h, float lr, float c1, float c2) {
+    if (threadIdx.x == 0) {
+        h[0] = lr;
+        h[1] = c1;
+        h[2] = c2;
+    }
+}
+
+__global__ void zero_scalar(float *__restrict__ p) {
+    if (threadIdx.x == 0) *p = 0.0f;
+}
+
+// Adam on one layer: weights i < N*K (then their BF16 copies W [N][K] and
+// W^T [K][N]), biases after them; gradients as partial slabs (gW [S][N][K],
+// gb [Sb][N]); hyp = {lr, c1, c2}
+__global__ void adam_layer(float *__restrict__ W, const float *__restrict__ gW,
+                           float *__restrict__ mW, float *__restrict__ vW, float *__restrict__ b,
+                           const float *__restrict__ gb, float *__restrict__ mb,
+                           float *__restrict__ vb, int N, int K, int S, int Sb,
+                           const float *__restrict__ hyp, float b1, float b2, float eps,
+                           bf16 *__restrict__ Wb, bf16 *__restrict__ WT) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float gi = g[i];
-    const float mi = b1 * m[i] + (1.0f - b1) * gi;
-    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    const float lr = hyp[0], c1 = hyp[1], c2 = hyp[2];
+    const int64_t nw = (int64_t)N * K;
+    if (i >= nw + N) return;
+    float *p = i < nw ? W : b;
+    const float *g = i < nw ? gW : gb;
+    float *m = i < nw ? mW : mb;
+    float *v = i < nw ? vW : vb;
+    const int64_t j = i < nw ? i : i - nw;
+    // the gradient: the sum of the partials (S split slabs of the weights,
+    // Sb 64-row blocks of the biases), in order
+    const int64_t stride = i < nw ? nw : N;
+    const int np = i < nw ? S : Sb;
+    float gi = 0.0f;
+    for (int z0 = 0; z0 < np; z0 += 16) {
+        // sixteen loads in flight, then their sum in order
+        float v[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = z0 + q < np ? g[(int64_t)(z0 + q) * stride + j] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            if (z0 + q < np) gi = z0 + q == 0 ? v[q] : gi + v[q];
+    }
+    const float mi = b1 * m[j] + (1.0f - b1) * gi;
+    const float vi = b2 * v[j] + (1.0f - b2) * gi * gi;
+    m[j] = mi;
+    v[j] = vi;
+    const float x = p[j] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[j] = x;
+    if (i < nw) {
+        const int n = (int)(i / K), k = (int)(i - (int64_t)n * K);
+        const bf16 h = to_bf16(x);
+        Wb[i] = h;
+        WT[(int64_t)k * N + n] = h;
+    }
 }
 
 // FP32 master W [N][K] -> BF16 W [N][K] and W^T [K][N]
@@ -306,9 +426,17 @@ struct Layer {
 struct winds_mlp {
     int n_in = 0, n_out = 0, hidden = 0, nl = 0, Bmax = 0;
     std::vector<Layer> L;
-    Buf Xb, XT, dA, loss;
+    Buf Xb, XT, loss, hyp;
     int64_t step = 0;
     hipStream_t st = nullptr;
+    // the training step as a graph, replayed while the batch pointers, the
+    // batch size and the stream repeat
+    const float *gX = nullptr, *gT = nullptr;
+    int gB = 0;
+    hipStream_t gst = nullptr;
+    int repeats = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
 };
 
 namespace {
@@ -322,13 +450,22 @@ using winds::set_error;
             return set_error(WINDS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-void gemm(hipStream_t st, const bf16 *A, int lda, const bf16 *B, int ldb, float *C, int ldc, int M,
-          int N, int K) {
-    dim3 grid((N + 63) / 64, (M + 63) / 64);
-    gemm_nt<<<grid, 256, 0, st>>>(A, lda, B, ldb, C, ldc, M, N, K);
+template <int EPI>
+void gemm(hipStream_t st, const bf16 *A, int lda, const bf16 *B, int ldb, int M, int N, int K,
+          const EpiArgs &e, int splits = 1, int64_t slab = 0) {
+    const int kspan = splits > 1 ? rup((K + splits - 1) / splits, kSlice) : K;
+    dim3 grid((N + kTile - 1) / kTile, (M + kTile - 1) / kTile, splits);
+    gemm_bf16<EPI><<<grid, 256, 0, st>>>(A, lda, B, ldb, M, N, K, kspan, slab, e);
 }
 
-// forward with the caches training needs (keep_t: also the [feature][batch] layouts)
+// K splits of the weight-gradient product over a batch of Bp rows: enough
+// workgroups for the chip at 256 x 256 weights, each split at least 256 rows
+int splits_of(int Bp) { return std::max(1, std::min(kMaxSplit, Bp / 256)); }
+
+EpiArgs epi_f32(float *C, int ldc) { return EpiArgs{C, ldc, nullptr, nullptr, 0, nullptr, 0, nullptr, 0}; }
+
+// forward with the caches training needs (keep_t: also the [feature][batch]
+// layouts and the pre-activations)
 int forward(winds_mlp *m, const float *X, int B, int Bp, bool keep_t, hipStream_t st) {
     Layer &l0 = m->L[0];
     if (keep_t)
@@ -341,16 +478,15 @@ int forward(winds_mlp *m, const float *X, int B, int Bp, bool keep_t, hipStream_
     const bf16 *in = m->Xb.as<bf16>();
     for (int i = 0; i < m->nl; ++i) {
         Layer &l = m->L[i];
-        gemm(st, in, l.Kp, l.Wb.as<bf16>(), l.Kp, l.Z.as<float>(), l.Np, Bp, l.Np, l.Kp);
-        const bool hidden = i + 1 < m->nl;
-        if (hidden && keep_t)
-            tile_pack<1><<<dim3((l.Np + 63) / 64, Bp / 64), 256, 0, st>>>(
-                l.Z.as<float>(), nullptr, l.b.as<float>(), Bp, l.Np, Bp, l.Np, l.Ab.as<bf16>(),
-                l.AT.as<bf16>());
-        else
-            bias_act<<<nblk((int64_t)Bp * l.Np), 256, 0, st>>>(
-                l.Z.as<float>(), l.b.as<float>(), Bp, l.Np, hidden ? 1 : 0, l.Ab.as<bf16>(),
-                nullptr);
+        if (i + 1 < m->nl) {
+            const EpiArgs e{keep_t ? l.Z.as<float>() : nullptr, l.Np, l.b.as<float>(), nullptr, 0,
+                            l.Ab.as<bf16>(), l.Np, keep_t ? l.AT.as<bf16>() : nullptr, Bp};
+            gemm<kEpiBiasSilu>(st, in, l.Kp, l.Wb.as<bf16>(), l.Kp, Bp, l.Np, l.Kp, e);
+        } else {
+            EpiArgs e = epi_f32(l.Z.as<float>(), l.Np);
+            e.bias = l.b.as<float>();
+            gemm<kEpiBias>(st, in, l.Kp, l.Wb.as<bf16>(), l.Kp, Bp, l.Np, l.Kp, e);
+        }
         MLP_HIP(hipGetLastError());
         in = l.Ab.as<bf16>();
     }
@@ -362,26 +498,53 @@ int backward(winds_mlp *m, const float *T, int B, int Bp, hipStream_t st, float 
     mse_grad<<<nblk((int64_t)Bp * lo.Np), 256, 0, st>>>(lo.Z.as<float>(), T, B, m->n_out, Bp, lo.Np,
                                                        lo.dZ.as<float>(), loss);
     MLP_HIP(hipGetLastError());
+    // the output layer's delta in BF16; the hidden layers' come from the
+    // input-gradient GEMM's epilogue
+    tile_pack<2><<<dim3((lo.Np + 63) / 64, Bp / 64), 256, 0, st>>>(
+        lo.dZ.as<float>(), nullptr, nullptr, Bp, lo.Np, Bp, lo.Np, lo.Db.as<bf16>(),
+        lo.DT.as<bf16>());
+    MLP_HIP(hipGetLastError());
     for (int i = m->nl - 1; i >= 0; --i) {
         Layer &l = m->L[i];
-        tile_pack<2><<<dim3((l.Np + 63) / 64, Bp / 64), 256, 0, st>>>(
-            l.dZ.as<float>(), nullptr, nullptr, Bp, l.Np, Bp, l.Np, l.Db.as<bf16>(),
-            l.DT.as<bf16>());
-        MLP_HIP(hipGetLastError());
         const bf16 *xt = i == 0 ? m->XT.as<bf16>() : m->L[i - 1].AT.as<bf16>();
-        // dW [Np][Kp] = dZ^T [Np][Bp] * X^T [Kp][Bp]^T
-        gemm(st, l.DT.as<bf16>(), Bp, xt, Bp, l.gW.as<float>(), l.Kp, l.Np, l.Kp, Bp);
-        col_sum<<<l.Np, 256, 0, st>>>(l.dZ.as<float>(), Bp, l.Np, l.gb.as<float>());
+        // dW [Np][Kp] = dZ^T [Np][Bp] * X^T [Kp][Bp]^T, as S partial slabs
+        // over the batch (S = splits_of(Bp)); db as one partial per 64 rows
+        const int S = splits_of(Bp);
+        gemm<kEpiF32>(st, l.DT.as<bf16>(), Bp, xt, Bp, l.Np, l.Kp, Bp,
+                      epi_f32(l.gW.as<float>(), l.Kp), S, (int64_t)l.Np * l.Kp);
+        col_sum_part<<<dim3((l.Np + 63) / 64, Bp / 64), 256, 0, st>>>(l.dZ.as<float>(), Bp, l.Np,
+                                                                      l.gb.as<float>());
         MLP_HIP(hipGetLastError());
         if (i > 0) {
             Layer &p = m->L[i - 1];
-            // dA [Bp][Kp] = dZ [Bp][Np] * W [Np][Kp]  (B operand W^T [Kp][Np])
-            gemm(st, l.Db.as<bf16>(), l.Np, l.WT.as<bf16>(), l.Np, m->dA.as<float>(), l.Kp, Bp,
-                 l.Kp, l.Np);
-            silu_back<<<nblk((int64_t)Bp * p.Np), 256, 0, st>>>(m->dA.as<float>(), p.Z.as<float>(),
-                                                               (int64_t)Bp * p.Np, p.dZ.as<float>());
+            // dZ_prev [Bp][Kp] = (dZ [Bp][Np] * W [Np][Kp]) SiLU'(Z_prev)
+            // (B operand W^T [Kp][Np])
+            const EpiArgs e{p.dZ.as<float>(), p.Np, nullptr, p.Z.as<float>(), p.Np,
+                            p.Db.as<bf16>(), p.Np, p.DT.as<bf16>(), Bp};
+            gemm<kEpiSiluBack>(st, l.Db.as<bf16>(), l.Np, l.WT.as<bf16>(), l.Np, Bp, l.Kp, l.Np,
+                               e);
             MLP_HIP(hipGetLastError());
         }
+    }
+    return WINDS_OK;
+}
+
+// One training step on stream st: the loss into the network's own scalar,
+// forward, backward, Adam with the scalars set_hyper stored.  Launches only,
+// so that it can be recorded into a graph.
+int train_body(winds_mlp *m, const float *X, const float *T, int B, hipStream_t st) {
+    const int Bp = rup(B, 64);
+    zero_scalar<<<1, 64, 0, st>>>(m->loss.as<float>());
+    int rc = forward(m, X, B, Bp, true, st);
+    if (!rc) rc = backward(m, T, B, Bp, st, m->loss.as<float>());
+    if (rc) return rc;
+    for (Layer &l : m->L) {
+        const int64_t nw = (int64_t)l.Np * l.Kp;
+        adam_layer<<<nblk(nw + l.Np), 256, 0, st>>>(
+            l.W.as<float>(), l.gW.as<float>(), l.mW.as<float>(), l.vW.as<float>(), l.b.as<float>(),
+            l.gb.as<float>(), l.mb.as<float>(), l.vb.as<float>(), l.Np, l.Kp, splits_of(Bp), Bp / 64,
+            m->hyp.as<const float>(), 0.9f, 0.999f, 1e-8f, l.Wb.as<bf16>(), l.WT.as<bf16>());
+        MLP_HIP(hipGetLastError());
     }
     return WINDS_OK;
 }
@@ -408,10 +571,10 @@ extern "C" {
 
 void winds_mlp_destroy(winds_mlp *m) {
     if (!m) return;
-    if (m->st) {
-        (void)hipStreamSynchronize(m->st);
-        (void)hipStreamDestroy(m->st);
-    }
+    (void)hipDeviceSynchronize();
+    if (m->exec) (void)hipGraphExecDestroy(m->exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    if (m->st) (void)hipStreamDestroy(m->st);
     delete m;
 }
 
@@ -443,21 +606,19 @@ int winds_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_hidden_layers, in
     MC(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
     const int Bp = m->Bmax;
     uint64_t s = seed;
-    int maxK = 0;
     for (int i = 0; i < m->nl; ++i) {
         Layer &l = m->L[i];
         l.K = i == 0 ? n_in : n_hidden;
         l.N = i + 1 < m->nl ? n_hidden : n_out;
         l.Kp = rup(l.K, 32);
         l.Np = rup(l.N, 32);
-        maxK = std::max(maxK, l.Kp);
         const size_t w = (size_t)l.Np * l.Kp * 4, b = (size_t)l.Np * 4;
         MC(l.W.alloc(w));
         MC(l.b.alloc(b));
         MC(l.Wb.alloc(w / 2));
         MC(l.WT.alloc(w / 2));
-        MC(l.gW.alloc(w));
-        MC(l.gb.alloc(b));
+        MC(l.gW.alloc(w * splits_of(Bp)));  // split partial slabs
+        MC(l.gb.alloc(b * (Bp / 64)));
         MC(l.mW.alloc(w));
         MC(l.vW.alloc(w));
         MC(l.mb.alloc(b));
@@ -488,8 +649,8 @@ int winds_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_hidden_layers, in
     }
     MC(m->Xb.alloc((size_t)Bp * m->L[0].Kp * 2));
     MC(m->XT.alloc((size_t)Bp * m->L[0].Kp * 2));
-    MC(m->dA.alloc((size_t)Bp * maxK * 4));
     MC(m->loss.alloc(256));
+    MC(m->hyp.alloc(256));
 #undef MC
     int rc = refresh(m, m->st);
     if (!rc && hipStreamSynchronize(m->st) != hipSuccess) rc = set_error(WINDS_E_HIP, "sync");
@@ -571,23 +732,50 @@ int winds_mlp_train_step(winds_mlp *m, const float *X, const float *T, int32_t B
     if (!m || !X || !T) return set_error(WINDS_E_ARG, "null argument");
     if (B < 1 || B > m->Bmax) return set_error(WINDS_E_DIMS, "batch out of range");
     hipStream_t st = (hipStream_t)stream;  // NULL: the default stream
-    const int Bp = rup(B, 64);
-    if (loss) MLP_HIP(hipMemsetAsync(loss, 0, sizeof(float), st));
-    int rc = forward(m, X, B, Bp, true, st);
-    if (!rc) rc = backward(m, T, B, Bp, st, loss);
-    if (rc) return rc;
     m->step++;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.0f - std::pow(b1, (float)m->step), c2 = 1.0f - std::pow(b2, (float)m->step);
-    for (Layer &l : m->L) {
-        const int64_t nw = (int64_t)l.Np * l.Kp;
-        adam<<<nblk(nw), 256, 0, st>>>(l.W.as<float>(), l.gW.as<float>(), l.mW.as<float>(),
-                                       l.vW.as<float>(), nw, lr, b1, b2, 1e-8f, c1, c2);
-        adam<<<nblk(l.Np), 256, 0, st>>>(l.b.as<float>(), l.gb.as<float>(), l.mb.as<float>(),
-                                         l.vb.as<float>(), l.Np, lr, b1, b2, 1e-8f, c1, c2);
-        MLP_HIP(hipGetLastError());
+    set_hyper<<<1, 64, 0, st>>>(m->hyp.as<float>(), lr, c1, c2);
+    MLP_HIP(hipGetLastError());
+    const bool same = X == m->gX && T == m->gT && B == m->gB && st == m->gst;
+    if (!same) {
+        if (m->exec) MLP_HIP(hipGraphExecDestroy(m->exec));
+        if (m->graph) MLP_HIP(hipGraphDestroy(m->graph));
+        m->exec = nullptr;
+        m->graph = nullptr;
+        m->gX = X;
+        m->gT = T;
+        m->gB = B;
+        m->gst = st;
+        m->repeats = 0;
+    } else {
+        m->repeats++;
     }
-    return refresh(m, st);
+    if (!m->exec && m->repeats >= 1) {
+        // the second call with the same batch buffers: record the step (on
+        // the network's own stream, which capture allows; nothing runs)
+        MLP_HIP(hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal));
+        int rc = train_body(m, X, T, B, m->st);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(m->st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return set_error(WINDS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        m->graph = g;
+        MLP_HIP(hipGraphInstantiateWithFlags(&m->exec, m->graph, 0));
+    }
+    if (m->exec) {
+        MLP_HIP(hipGraphLaunch(m->exec, st));
+    } else {
+        const int rc = train_body(m, X, T, B, st);
+        if (rc) return rc;
+    }
+    if (loss)
+        MLP_HIP(hipMemcpyAsync(loss, m->loss.p, sizeof(float), hipMemcpyDeviceToDevice, st));
+    return WINDS_OK;
 }
 
 int winds_mlp_gradients(winds_mlp *m, const float *X, const float *T, int32_t B, float *grads,
@@ -601,10 +789,24 @@ int winds_mlp_gradients(winds_mlp *m, const float *X, const float *T, int32_t B,
     if (!rc) rc = backward(m, T, B, Bp, m->st, m->loss.as<float>());
     if (rc) return rc;
     MLP_HIP(hipStreamSynchronize(m->st));
+    const int S = splits_of(Bp);
     for (const Layer &l : m->L) {
-        std::vector<float> w((size_t)l.Np * l.Kp), b(l.Np);
-        MLP_HIP(hipMemcpy(w.data(), l.gW.p, w.size() * 4, hipMemcpyDeviceToHost));
-        MLP_HIP(hipMemcpy(b.data(), l.gb.p, b.size() * 4, hipMemcpyDeviceToHost));
+        // the split partials added in split order, as the update does
+        const size_t nw = (size_t)l.Np * l.Kp;
+        const int Sb = Bp / 64;
+        std::vector<float> wp(nw * S), bp((size_t)l.Np * Sb), w(nw), b(l.Np);
+        MLP_HIP(hipMemcpy(wp.data(), l.gW.p, wp.size() * 4, hipMemcpyDeviceToHost));
+        MLP_HIP(hipMemcpy(bp.data(), l.gb.p, bp.size() * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nw; ++i) {
+            float g = wp[i];
+            for (int z = 1; z < S; ++z) g += wp[z * nw + i];
+            w[i] = g;
+        }
+        for (int n = 0; n < l.Np; ++n) {
+            float g = bp[n];
+            for (int z = 1; z < Sb; ++z) g += bp[(size_t)z * l.Np + n];
+            b[n] = g;
+        }
         for (int n = 0; n < l.N; ++n)
             for (int k = 0; k < l.K; ++k) *grads++ = w[(size_t)n * l.Kp + k];
         for (int n = 0; n < l.N; ++n) *grads++ = b[n];
