@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -69,7 +70,7 @@ struct DevBuf {
             if (e != hipSuccess) return e;
         }
         size_t want = need < 4096 ? 4096 : need;
-        hipError_t e = hipMalloc(&p, want);
+        hipError_t e = winds::dev_alloc(&p, want);
         if (e == hipSuccess) bytes = want;
         return e;
     }
@@ -106,6 +107,17 @@ int check_device(int dev) {
 int cu_count() { return winds::device_cu_count(); }
 
 }  // namespace
+
+hipError_t winds::dev_alloc(void **p, size_t bytes) {
+    if (const char *v = std::getenv("WINDS_FAIL_ALLOC_BYTES")) {
+        const long long lim = std::atoll(v);
+        if (lim > 0 && bytes >= (unsigned long long)lim) {
+            *p = nullptr;
+            return hipErrorOutOfMemory;
+        }
+    }
+    return hipMalloc(p, bytes);
+}
 
 int winds::device_cu_count() {
     static int cached = 0;

@@ -346,10 +346,10 @@ int build_plan(winds_batch *b) {
     std::vector<winds::WakeSpan> spans(b->n_wakes);
     for (int w = 0; w < b->n_wakes; ++w)
         spans[w] = winds::WakeSpan{b->hw[w].t_off, b->hw[w].nt, b->hw[w].s_off, b->hw[w].ns};
-    WB_HIP(hipMalloc(&b->d_wakes, sizeof(BatchWake) * std::max(1, b->n_wakes)));
-    WB_HIP(hipMalloc(&b->d_spans, sizeof(winds::WakeSpan) * std::max(1, b->n_wakes)));
-    WB_HIP(hipMalloc(&b->d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
-    WB_HIP(hipMalloc(&b->d_counter, 256));
+    WB_HIP(winds::dev_alloc((void **)&b->d_wakes, sizeof(BatchWake) * std::max(1, b->n_wakes)));
+    WB_HIP(winds::dev_alloc((void **)&b->d_spans, sizeof(winds::WakeSpan) * std::max(1, b->n_wakes)));
+    WB_HIP(winds::dev_alloc((void **)&b->d_items, sizeof(WorkItem) * std::max<size_t>(1, items.size())));
+    WB_HIP(winds::dev_alloc((void **)&b->d_counter, 256));
     WB_HIP(hipMemcpy(b->d_wakes, b->hw.data(), sizeof(BatchWake) * b->n_wakes, hipMemcpyHostToDevice));
     WB_HIP(hipMemcpy(b->d_spans, spans.data(), sizeof(winds::WakeSpan) * b->n_wakes,
                      hipMemcpyHostToDevice));
@@ -628,7 +628,7 @@ int winds_batch_autotune(winds_batch *b, const double *targets, const double *se
             if (need > ws_bytes) {
                 if (ws) (void)hipFree(ws);
                 ws = nullptr;
-                if (hipMalloc(&ws, need) != hipSuccess) {
+                if (winds::dev_alloc((void **)&ws, need) != hipSuccess) {
                     ws_bytes = 0;
                     continue;  // candidate does not fit in memory: skip it
                 }

@@ -1,6 +1,8 @@
 // bs_common.hpp -- error reporting shared by the C-ABI translation units.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <string>
 
 namespace winds {
@@ -9,6 +11,10 @@ int set_error(int code, const std::string &msg);
 void clear_error();
 // Compute units of the current device (cached; 256 if unknown).
 int device_cu_count();
+// Every device allocation of the library.  When the environment variable
+// WINDS_FAIL_ALLOC_BYTES holds a positive number, requests of at least that
+// many bytes fail with hipErrorOutOfMemory instead (tests of the error paths).
+hipError_t dev_alloc(void **p, size_t bytes);
 }  // namespace winds
 
 #include "bs_kernels.hpp"

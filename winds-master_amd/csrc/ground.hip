@@ -284,9 +284,9 @@ int winds_ground_create(int32_t np_side, double x_min, double x_max, double y_mi
             return fail(set_error(WINDS_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_))); \
     } while (0)
     WGC(hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking));
-    WGC(hipMalloc(&g->A, (size_t)N * N * sizeof(double)));
-    WGC(hipMalloc(&g->nrm, (size_t)3 * N * sizeof(double)));
-    WGC(hipMalloc(&g->work, ((size_t)5 * N + winds::gnd::kS) * sizeof(double)));
+    WGC(winds::dev_alloc((void **)&g->A, (size_t)N * N * sizeof(double)));
+    WGC(winds::dev_alloc((void **)&g->nrm, (size_t)3 * N * sizeof(double)));
+    WGC(winds::dev_alloc((void **)&g->work, ((size_t)5 * N + winds::gnd::kS) * sizeof(double)));
     winds::gnd::k_normals<<<blocks(N), 256, 0, g->st>>>(g->g, g->nrm);
     WGC(hipGetLastError());
     winds::gnd::k_matrix<<<blocks((int64_t)N * N), 256, 0, g->st>>>(g->g, g->nrm, g->A);

@@ -789,7 +789,7 @@ struct DBuf {
         if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
-        hipError_t e = hipMalloc(&p, b < 256 ? 256 : b);
+        hipError_t e = winds::dev_alloc((void **)&p, b < 256 ? 256 : b);
         if (e == hipSuccess) bytes = b;
         return e;
     }

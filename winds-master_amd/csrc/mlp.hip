@@ -408,7 +408,7 @@ struct Buf {
     hipError_t alloc(size_t b) {
         if (p) (void)hipFree(p);
         p = nullptr;
-        return hipMalloc(&p, b < 256 ? 256 : b);
+        return winds::dev_alloc((void **)&p, b < 256 ? 256 : b);
     }
     template <class T>
     T *as() const { return (T *)p; }
