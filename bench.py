@@ -231,11 +231,24 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
     held = S.build_training_set(50_000, o, seed=8, split=sc.split, target_ages=sc.target_ages)
     rows = 31 * sc.target_ages * 3 + 30 * 3
     net = MLP(S.NIN, MLP_HIDDEN, MLP_LAYERS, S.NOUT, max(batch, rows), seed=7)
+    pre = {}
     if march_set is not None:
-        in_norm, out_norm, (s1, s2) = S.train_two_phase(net, ts, march_set, train_steps, batch,
-                                                        log=log)
+        def pretrained(net_, inn, onn):
+            # the network after the synthetic phase only: how it generalises
+            # to held-out synthetic wakes and to the march it has not seen
+            Hn_, _ = S.normalise(held.F, None, inn, onn)
+            Fm_, _ = S.normalise(march_set.F, None, inn, onn)
+            Yh, Ym = S.predict(net_, Hn_, onn), S.predict(net_, Fm_, onn)
+            pre.update({
+                "normwise_err_synthetic_heldout": S.normwise_error(Yh, held.Y),
+                "rms_err_synthetic_heldout": S.rms_error(Yh, held.Y),
+                "normwise_err_march_samples": S.normwise_error(Ym, march_set.Y),
+                "rms_err_march_samples": S.rms_error(Ym, march_set.Y)})
+            log(f"  after the synthetic phase: {pre}")
+        in_norm, out_norm, (s1, s2), (l1, l2) = S.train_two_phase(
+            net, ts, march_set, train_steps, batch, log=log, between=pretrained)
         secs = s1 + s2
-        loss = None
+        loss = {"synthetic_phase": l1, "march_phase": l2}
     else:
         in_norm, out_norm = S.fit_norm(ts.F, ts.Y)
         Fn, Yn = S.normalise(ts.F, ts.Y, in_norm, out_norm)
@@ -247,15 +260,19 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
     train_flop = (6 * W - 2 * S.NIN * MLP_HIDDEN) * batch
     train_tflops = train_flop * train_steps / secs / 1e12
     Ft, _ = S.normalise(ts.F[:200_000].contiguous(), None, in_norm, out_norm)
-    err_train = S.normwise_error(S.predict(net, Ft, out_norm), ts.Y[:200_000])
-    err_held = S.normwise_error(S.predict(net, Hn, out_norm), held.Y)
-    err_march = None
+    Yt, Yh = S.predict(net, Ft, out_norm), S.predict(net, Hn, out_norm)
+    err_train = S.normwise_error(Yt, ts.Y[:200_000])
+    err_held = S.normwise_error(Yh, held.Y)
+    rms_train, rms_held = S.rms_error(Yt, ts.Y[:200_000]), S.rms_error(Yh, held.Y)
+    err_march = rms_march = None
     if march_set is not None:
         Fm, _ = S.normalise(march_set.F, None, in_norm, out_norm)
-        err_march = S.normwise_error(S.predict(net, Fm, out_norm), march_set.Y)
+        Ym = S.predict(net, Fm, out_norm)
+        err_march, rms_march = S.normwise_error(Ym, march_set.Y), S.rms_error(Ym, march_set.Y)
     log(f"surrogate training: {train_steps} steps x {batch}: {secs:.1f} s "
         f"({train_tflops:.1f} TFLOP/s), loss {loss}, normwise error synthetic train "
-        f"{err_train:.3e} held-out {err_held:.3e} march samples {err_march}")
+        f"{err_train:.3e} held-out {err_held:.3e} march samples {err_march}; rms "
+        f"{rms_train:.3e} / {rms_held:.3e} / {rms_march}")
     # inference rate on max_batch rows
     B = net.max_batch
     Xb = Ft[:B].contiguous()
@@ -277,7 +294,7 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
         f"{inf_tflops:.1f} TFLOP/s")
     rep = {
         "network": f"{S.NIN} inputs -> {MLP_LAYERS} x {MLP_HIDDEN} SiLU -> 3, BF16 weights and "
-                   f"activations, FP32 accumulation (v_mfma_f32_32x32x8bf16_1k)",
+                   f"activations, FP32 accumulation (v_mfma_f32_32x32x16_bf16)",
         "parameters": net.num_params(),
         "split_age": sc.split, "target_ages": sc.target_ages,
         "training": "half the steps on synthetic wakes, half on the march's own samples",
@@ -290,8 +307,13 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
         "train_samples_per_s": round(train_steps * batch / secs, 1),
         "train_tflops": round(train_tflops, 2),
         "final_loss": loss,
+        "error_metrics": "normwise = max |error| / max |label|; rms = rms |error| / rms |label| "
+                         "(3-vectors of far-wake velocity over U)",
         "normwise_err_synthetic_train": err_train, "normwise_err_synthetic_heldout": err_held,
         "normwise_err_march_samples": err_march,
+        "rms_err_synthetic_train": rms_train, "rms_err_synthetic_heldout": rms_held,
+        "rms_err_march_samples": rms_march,
+        "after_synthetic_phase_only": pre or None,
         "inference_rows": B, "inference_ms": round(fwd_ms, 4),
         "inference_samples_per_s": round(inf_rate, 1),
         "inference_tflops": round(inf_tflops, 2),

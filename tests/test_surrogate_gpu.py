@@ -172,7 +172,7 @@ def small_net(developed):
     tg = S.build_training_set(100_000, o, seed=7, split=72, target_ages=36, max_ages=200, nb=2,
                               R=20.0, U=10.0, hub=1.0, nst_range=(7, 7), dpsi_deg=10.0)
     net = MLP(S.NIN, 256, 4, S.NOUT, 2048, seed=7)
-    in_norm, out_norm, secs = S.train_two_phase(net, tg, tm, steps=8000, batch=1024)
+    in_norm, out_norm, secs, _ = S.train_two_phase(net, tg, tm, steps=8000, batch=1024)
     Fm, _ = S.normalise(tm.F, None, in_norm, out_norm)
     err = S.normwise_error(S.predict(net, Fm, out_norm), tm.Y)
     flush_print(f"[surrogate pretrain] {tg.wakes} synthetic wakes + {tm.wakes} march snapshots, "

@@ -335,20 +335,24 @@ def train(net, Fn, Yn, steps: int, batch: int, lr: float = 1e-3, lr_end: float =
 
 
 def train_two_phase(net, synthetic: TrainingSet, march: TrainingSet, steps: int, batch: int,
-                    log=None):
+                    log=None, between=None):
     """Normalisation from both sets; half the steps on the synthetic wakes
     (rate 1e-3 -> 3e-5), half on the march's own samples (3e-4 -> 1e-5).
-    Returns (in_norm, out_norm, seconds of each phase)."""
+    between(net, in_norm, out_norm), if given, runs after the first phase.
+    Returns (in_norm, out_norm, seconds of each phase, final mean loss of
+    each phase)."""
     both = concat(synthetic, march)
     in_norm, out_norm = fit_norm(both.F, both.Y)
     del both
     Fs, Ys = normalise(synthetic.F, synthetic.Y, in_norm, out_norm)
-    s1, _ = train(net, Fs, Ys, steps=steps // 2, batch=batch, lr=1e-3, lr_end=3e-5, log=log)
+    s1, l1 = train(net, Fs, Ys, steps=steps // 2, batch=batch, lr=1e-3, lr_end=3e-5, log=log)
     del Fs, Ys
+    if between is not None:
+        between(net, in_norm, out_norm)
     Fm, Ym = normalise(march.F, march.Y, in_norm, out_norm)
-    s2, _ = train(net, Fm, Ym, steps=steps - steps // 2, batch=batch, lr=3e-4, lr_end=1e-5,
-                  log=log)
-    return in_norm, out_norm, (s1, s2)
+    s2, l2 = train(net, Fm, Ym, steps=steps - steps // 2, batch=batch, lr=3e-4, lr_end=1e-5,
+                   log=log)
+    return in_norm, out_norm, (s1, s2), (l1, l2)
 
 
 def predict(net, Fn, out_norm, chunk: int | None = None):
@@ -365,6 +369,14 @@ def normwise_error(Yp, Y):
     import torch
     e = torch.linalg.vector_norm((Yp - Y).double(), dim=1).max()
     r = torch.linalg.vector_norm(Y.double(), dim=1).max()
+    return float(e / r)
+
+
+def rms_error(Yp, Y):
+    """sqrt(mean |y_i - y_ref_i|^2) / sqrt(mean |y_ref_i|^2) over the 3-vectors."""
+    import torch
+    e = ((Yp - Y).double() ** 2).sum(1).mean().sqrt()
+    r = (Y.double() ** 2).sum(1).mean().sqrt()
     return float(e / r)
 
 
