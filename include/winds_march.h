@@ -12,7 +12,7 @@
  *                     30-51, core/filamentmod.m:27-41), shed + trailing
  *                     induced velocity at every node (the BiotSavartMex calls
  *                     of WInDS.m:150-154), integrator (numerical/pcc.m:31-75,
- *                     fe.m:30-31, ab4.m:34-42), Kutta-Joukowski fixed point
+ *                     fe.m:30-31, ab2.m:34-40, ab4.m:34-42), Kutta-Joukowski fixed point
  *                     in FP64 with each station's own airfoil table
  *                     (core/KuttaJoukowski.m:42-49, 173-298) and rotor
  *                     thrust/torque/power (postproc/perform.m:16-62).
@@ -42,6 +42,7 @@ extern "C" {
 #define WINDS_INTEG_PCC 0 /* numerical/pcc.m */
 #define WINDS_INTEG_FE 1  /* numerical/fe.m  */
 #define WINDS_INTEG_AB4 2 /* numerical/ab4.m */
+#define WINDS_INTEG_AB2 3 /* numerical/ab2.m (forward Euler at the first step) */
 
 typedef struct winds_march winds_march;
 

@@ -14,7 +14,10 @@ files literally:
   vcore            core/vcore.m:30-51 (global time fastout.Time, not age)
   filamentmod      core/filamentmod.m:27-41 (sqrt of the already-square-rooted
                    lengths, strain between equal age indices of two times)
-  fe / pcc / ab4   numerical/fe.m:30-31, pcc.m:31-75, ab4.m:34-42
+  fe / pcc / ab2 / ab4  numerical/fe.m:30-31, pcc.m:31-75, ab2.m:34-40,
+                   ab4.m:34-42 (ab2.m's forward-Euler start passes dt where
+                   fe.m takes user, an error in MATLAB; here the start is
+                   fe.m's, as ab4.m's is)
   Kutta-Joukowski  core/KuttaJoukowski.m:42-49 (fixed point) and kj() :173-298,
                    each station interpolating its own airfoil's table
                    (blade.NFoil, :262-269), other sources at the lifting line
@@ -149,7 +152,8 @@ class OracleMarch:
 
     def _advect(self, dom, vel, p_next, vel_next=None, hist=None):
         """Nodes of time index p_next: ages 0 and 1 from the tables, older ages
-        moved by fe.m, pcc.m's corrector (vel_next) or ab4.m (hist)."""
+        moved by fe.m, pcc.m's corrector (vel_next), ab4.m (hist of four
+        velocity fields) or ab2.m (hist of two)."""
         A_new = min(dom.shape[2] + 1, self.max_ages)
         q, tr = self.quarter_trail(p_next)
         out = np.empty((self.nst, 3, A_new, self.nb))
@@ -159,6 +163,12 @@ class OracleMarch:
         src = dom[:, :, 1:1 + n, :]
         if vel_next is not None:       # pcc corrector
             out[:, :, 2:, :] = src + self.dt / 2 * (vel_next[:, :, 2:, :] + vel[:, :, 1:1 + n, :])
+        elif hist is not None and len(hist) == 2:   # ab2
+            v1 = hist[1]
+            sh = np.zeros_like(vel[:, :, 1:1 + n, :])
+            if n > 1:
+                sh[:, :, 1:, :] = v1[:, :, 1:n, :]
+            out[:, :, 2:, :] = src + self.dt / 2 * (3 * vel[:, :, 1:1 + n, :] - sh)
         elif hist is not None:         # ab4
             v0, v1, v2, v3 = hist
             acc = 55 * vel[:, :, 1:1 + n, :]
@@ -212,6 +222,8 @@ class OracleMarch:
         elif self.integ == "ab4" and p >= 4:
             h = self.vel_hist
             dom_next = self._advect(self.dom, vel, p + 1, hist=(vel, h[-1], h[-2], h[-3]))
+        elif self.integ == "ab2" and p >= 2:
+            dom_next = self._advect(self.dom, vel, p + 1, hist=(vel, self.vel_hist[-1]))
         else:
             dom_next = self._advect(self.dom, vel, p + 1)
         # Kutta-Joukowski fixed point at time p

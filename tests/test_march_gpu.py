@@ -76,7 +76,8 @@ def compare(m, om, label):
 
 
 @pytest.mark.parametrize("sem,d,integ", [("mex", 1e-4, "pcc"), ("matlab", "visc2", "pcc"),
-                                         ("mex", 1e-4, "ab4"), ("matlab", 1e-4, "fe")])
+                                         ("mex", 1e-4, "ab2"), ("mex", 1e-4, "ab4"),
+                                         ("matlab", 1e-4, "fe")])
 def test_march_fp64_matches_oracle(need_gpu, sem, d, integ):
     steps, max_ages = 14, 10        # the wake hits max_ages after 8 steps
     m, om = run_pair(small_tables(), max_ages, steps, sem, d, integ)

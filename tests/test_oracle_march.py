@@ -11,7 +11,8 @@
   a node leaves it) and each age's trailing circulation sums to zero;
 - advection (pcc.m): with no lift there is no circulation, and every node
   moves with the free stream exactly; in a rigid rotation field the node
-  updates of fe.m, pcc.m and ab4.m are first-, second- and fourth-order
+  updates of fe.m, pcc.m, ab2.m and ab4.m are first-, second-, second- and
+  fourth-order
   accurate (halving dt divides the error by 2, 4 and 16);
 - Kutta-Joukowski (KuttaJoukowski.m:42-49, :173-298) against Prandtl's
   lifting line for an elliptic wing (C_L = 2 pi alpha0 / (1 + 2/AR), uniform
@@ -203,6 +204,31 @@ def test_ab4_is_fourth_order():
         errs.append(np.max(np.abs(dom[:, :, a0 + k, :] - exact)))
     rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
     assert np.all(np.abs(rates - 4.0) <= 0.3), rates
+
+
+def test_ab2_is_second_order():
+    """Adams-Bashforth 2 (ab2.m:38-39) on nodes old enough to have an
+    earlier velocity, started from the exact history."""
+    om = OracleMarch(small_rotor(), 400)
+    rng = np.random.default_rng(5)
+    A0, a0 = 8, 5
+    dom0 = np.asfortranarray(rng.uniform(-10, 10, size=(om.nst, 3, A0, om.nb)))
+    omega, T = 2.0, 0.4
+    field = _rotation_field(omega)
+    exact = _rotate(dom0[:, :, a0, :], omega * T)
+    errs = []
+    for k in (16, 32, 64):
+        dt = T / k
+        om.dt = dt
+        prev = field(np.asfortranarray(_rotate(dom0[:, :, 1:, :], -omega * dt)))
+        dom = dom0.copy()
+        for _ in range(k):
+            v = field(dom)
+            dom = om._advect(dom, v, 2, hist=(v, prev))
+            prev = v
+        errs.append(np.max(np.abs(dom[:, :, a0 + k, :] - exact)))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(np.abs(rates - 2.0) <= 0.2), rates
 
 
 def _elliptic_wing(nst, alpha0_deg, hub=5.0, R=65.0, c0=7.5, U=10.0, far=1e6):

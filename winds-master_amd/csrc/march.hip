@@ -410,9 +410,11 @@ __global__ void k_add_uinf(int64_t n, double ux, double uy, double uz,
 
 // New ages 2.. from the old nodes: forward Euler (fe.m:31), PCC corrector
 // (pcc.m:74-75, vpred != null) or Adams-Bashforth 4 (ab4.m:38-42, h1 != null)
+// ab: 4 = ab4.m (h1..h3 the velocities of the three earlier time levels),
+// 2 = ab2.m (h1 only), 0 = fe.m, or pcc.m's corrector when vpred is given
 __global__ void k_advect(int nst, int nb, int A_old, int A_new, double dt,
                          const double *__restrict__ Xo, const double *__restrict__ vo,
-                         const double *__restrict__ vpred, const double *__restrict__ h1,
+                         const double *__restrict__ vpred, int ab, const double *__restrict__ h1,
                          int A1, const double *__restrict__ h2, int A2,
                          const double *__restrict__ h3, int A3, double *__restrict__ Xn) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -433,7 +435,15 @@ __global__ void k_advect(int nst, int nb, int A_old, int A_new, double dt,
         double x;
         if (vpred) {
             x = src + dt / 2 * (vpred[c * Nn + jn] + v0);
-        } else if (h1) {
+        } else if (ab == 2) {
+            // ab2.m:38-39: the earlier level's velocity one age younger
+            double sh = 0.0;
+            if (ai >= 1) {
+                const int64_t N1 = (int64_t)nst * A1 * nb;
+                sh = h1[c * N1 + m + (int64_t)ai * nst + (int64_t)b * nst * A1];
+            }
+            x = src + dt / 2 * (3 * v0 - sh);
+        } else if (ab == 4) {
             double acc = 55 * v0;
             const double *hs[3] = {h1, h2, h3};
             const int As[3] = {A1, A2, A3};
@@ -947,25 +957,27 @@ int vcore(winds_march *m, int p, const double *X, int A, const double *gsp, cons
 }
 
 int advect(winds_march *m, const double *Xo, int A_old, const double *vo, const double *vpred,
-           bool ab4, int p_next, double *Xn, int A_new) {
+           int ab, int p_next, double *Xn, int A_new) {
     k_nodes01<<<blocks((int64_t)m->g.nst * m->g.nb), 256, 0, m->st>>>(m->g, p_next - 1, A_new, Xn);
     WM_HIP(hipGetLastError());
     const int64_t n = (int64_t)m->g.nst * std::max(0, A_new - 2) * m->g.nb;
     if (n > 0) {
         const double *h1 = nullptr, *h2 = nullptr, *h3 = nullptr;
         int a1 = 0, a2 = 0, a3 = 0;
-        if (ab4) {
+        if (ab) {
             // the last pushed history entry is time p-1
             auto slot = [&](int k) { return (m->n_hist - k) % 3; };
             h1 = m->hist[slot(1)].d();
             a1 = m->hist_A[slot(1)];
-            h2 = m->hist[slot(2)].d();
-            a2 = m->hist_A[slot(2)];
-            h3 = m->hist[slot(3)].d();
-            a3 = m->hist_A[slot(3)];
+            if (ab == 4) {
+                h2 = m->hist[slot(2)].d();
+                a2 = m->hist_A[slot(2)];
+                h3 = m->hist[slot(3)].d();
+                a3 = m->hist_A[slot(3)];
+            }
         }
         k_advect<<<blocks(n), 256, 0, m->st>>>(m->g.nst, m->g.nb, A_old, A_new, m->cfg.dt, Xo, vo,
-                                               vpred, h1, a1, h2, a2, h3, a3, Xn);
+                                               vpred, ab, h1, a1, h2, a2, h3, a3, Xn);
         WM_HIP(hipGetLastError());
     }
     return WINDS_OK;
@@ -1172,7 +1184,7 @@ int setup(winds_march *m) {
     rc = push_hist(m, m->vel.d(), A);
     if (rc) return rc;
     const int A_new = std::min(A + 1, m->cfg.max_ages);
-    rc = advect(m, X, A, m->vel.d(), nullptr, false, 2, m->nodes[1].d(), A_new);
+    rc = advect(m, X, A, m->vel.d(), nullptr, 0, 2, m->nodes[1].d(), A_new);
     if (rc) return rc;
     m->cur = 1;
     m->A = A_new;
@@ -1215,9 +1227,11 @@ int step_once(winds_march *m) {
     double *Xn = m->nodes[nn].d();
     const int integ = m->cfg.integrator;
     if (integ == WINDS_INTEG_AB4 && p >= 4) {
-        rc = advect(m, X, A, m->vel.d(), nullptr, true, p + 1, Xn, A_new);
+        rc = advect(m, X, A, m->vel.d(), nullptr, 4, p + 1, Xn, A_new);
+    } else if (integ == WINDS_INTEG_AB2 && p >= 2) {
+        rc = advect(m, X, A, m->vel.d(), nullptr, 2, p + 1, Xn, A_new);
     } else {
-        rc = advect(m, X, A, m->vel.d(), nullptr, false, p + 1, Xn, A_new);
+        rc = advect(m, X, A, m->vel.d(), nullptr, 0, p + 1, Xn, A_new);
         if (!rc && integ == WINDS_INTEG_PCC && p > 2) {
             // pcc.m:40-75: Kelvin and vcore for p+1 at the predicted nodes,
             // induced velocity there, trapezoidal corrector
@@ -1231,7 +1245,7 @@ int step_once(winds_march *m) {
                                    t + 1, m->uind.d(),
                                    how == kSurrogate ? kSurrogate : kDirect);
             if (!rc) rc = add_wind(m, p + 1, Nn, m->uind.d(), m->velp.d());
-            if (!rc) rc = advect(m, X, A, m->vel.d(), m->velp.d(), false, p + 1, Xn, A_new);
+            if (!rc) rc = advect(m, X, A, m->vel.d(), m->velp.d(), 0, p + 1, Xn, A_new);
         }
     }
     if (rc) return rc;
@@ -1318,7 +1332,7 @@ int winds_march_create(const winds_march_cfg *cfg, winds_march **out) {
     for (int i = 0; i < c.nst - 1; ++i)
         if (c.foil[i] < 1 || c.foil[i] > c.n_foils)
             return set_error(WINDS_E_ARG, "blade.NFoil names a table that does not exist");
-    if (c.integrator < WINDS_INTEG_PCC || c.integrator > WINDS_INTEG_AB4)
+    if (c.integrator < WINDS_INTEG_PCC || c.integrator > WINDS_INTEG_AB2)
         return set_error(WINDS_E_MODE, "unknown integrator");
     if (!(c.dt > 0.0) || !(c.tip_radius > 0.0)) return set_error(WINDS_E_ARG, "bad dt/TipRad");
     winds_march *m = new winds_march();

@@ -7,7 +7,7 @@ import numpy as np
 
 from . import _lib
 
-INTEG = {"pcc": 0, "fe": 1, "ab4": 2}
+INTEG = {"pcc": 0, "fe": 1, "ab4": 2, "ab2": 3}
 
 
 class MarchCfg(C.Structure):
