@@ -413,6 +413,31 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
             log(f"  {name} {out[f'{name}_speedup_vs_fp32c']}x vs fp32c; "
                 f"{out[f'{name}_stats_steady']}")
         del f
+    # (2b) the same steps replayed from recorded HIP graphs (winds_march_set_graphs),
+    # and a short-wake march where launches weigh more: NREL 5-MW, 72 ages
+    def timed(f, warm, n):
+        f.step(warm)
+        f.sync()
+        t0 = time.perf_counter()
+        f.step(n)
+        f.sync()
+        return (time.perf_counter() - t0) / n * 1e3
+
+    f = fork(winds.PREC_FP32C)
+    f.set_graphs(True)
+    out["step_ms_fp32c_graph"] = round(timed(f, 9, 12), 2)
+    del f
+    short = cases.tables(cases.nrel_rotor(t=(0.0, 30.0, 10.0)))
+    g_ms = {}
+    for mode in (False, True):
+        f = March(short, 72, march_opts(winds.PREC_FP32C), integrator="pcc")
+        f.set_graphs(mode)
+        g_ms[mode] = timed(f, 90, 120)
+        del f
+    out["short_wake_step_ms"] = {"case": "NREL 5-MW, 18 stations, 72 ages, FP32C, PCC",
+                                 "eager": round(g_ms[False], 3), "graph": round(g_ms[True], 3)}
+    log(f"march step with graphs at 1440 ages: {out['step_ms_fp32c_graph']} ms; short wake "
+        f"{out['short_wake_step_ms']}")
     # (3) revs more revolutions from the developed state
     hist = {}
     plan = [("fp32c", winds.PREC_FP32C, None, steps)]

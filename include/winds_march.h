@@ -106,6 +106,16 @@ void winds_march_destroy(winds_march *m);
  * two rows of nodes there); beyond that the call fails. */
 int winds_march_step(winds_march *m, int32_t nsteps);
 
+/* on != 0: once the wake is at max_ages, without the surrogate, each step
+ * is recorded into a HIP graph the first time its phase of the buffer
+ * rotation (six phases) comes round, after two steady steps run eagerly,
+ * and replayed from then on (the step's time index, history row, r0, Time
+ * and free stream are read from device memory that a small kernel writes
+ * before each step).  The results are those of the eager steps, bit for
+ * bit.  Off by default; changing it, set_state and set_surrogate drop the
+ * recorded graphs. */
+int winds_march_set_graphs(winds_march *m, int32_t on);
+
 /* Wait for queued work; report a diverging Kutta-Joukowski solve
  * (NaN lift coefficient, KuttaJoukowski.m:272-274) as an error. */
 int winds_march_sync(winds_march *m);

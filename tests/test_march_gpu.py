@@ -169,3 +169,33 @@ def test_fast_file_set_matches_oracle(need_gpu, tmp_path):
     assert tb.ns == 17 and tb.nst == 18
     m, om = run_pair(tb, 12, 16, "mex", 1e-4, "pcc")
     compare(m, om, "FAST file set")
+
+
+@pytest.mark.parametrize("sem,d,integ,prec", [("mex", 1e-4, "pcc", winds.PREC_FP64),
+                                              ("mex", 1e-4, "ab4", winds.PREC_FP32C),
+                                              ("matlab", "visc2", "fe", winds.PREC_FP64),
+                                              ("mex", 1e-4, "ab2", winds.PREC_FP32C)])
+def test_graph_replayed_steps_equal_eager_steps(need_gpu, sem, d, integ, prec):
+    """winds_march_set_graphs: once the 9-age wake is steady, each of the
+    six phases of the buffer rotation is recorded once and replayed.  A
+    march with graphs and one without step side by side; between chunks of
+    steps the program does eager work of its own (state and history reads,
+    torch kernels); after every chunk nodes, circulation and the
+    performance history are equal bit for bit."""
+    import torch
+    from winds.march import March
+    flush_print(f"[march graphs] {sem} {d} {integ} precision {prec}")
+    tb = small_tables(60)
+    a = March(tb, 9, wake_opts(sem, d, prec), integrator=integ)
+    b = March(tb, 9, wake_opts(sem, d, prec), integrator=integ)
+    a.set_graphs(True)
+    scratch = torch.zeros(1000, device="cuda")
+    for chunk in (8, 3, 5, 7, 11, 13):       # 47 steps: growth, then ~7 rounds of 6 phases
+        a.step(chunk)
+        b.step(chunk)
+        scratch.add_(1.0)
+        na, gsa, gta = a.state()
+        nb, gsb, gtb = b.state()
+        assert np.array_equal(na, nb) and np.array_equal(gsa, gsb) and np.array_equal(gta, gtb)
+        assert np.array_equal(a.perf(), b.perf())
+    assert a.info()["p"] == b.info()["p"] == 49

@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -61,11 +62,31 @@ __device__ __forceinline__ double tab4(const double *t, int n, int nt, int i, in
     return t[i + (int64_t)n * (c + 3 * (p0 + (int64_t)nt * b))];
 }
 
+// Per-step values that a step's kernels read from device memory rather than
+// as arguments, so that every step of a steady march enqueues the same
+// launches and a recorded graph of them can be replayed.  k_set_step writes
+// them before the step's launches.
+struct StepDev {
+    int p0;             // table index of the step's time index p (p - 1)
+    int row;            // performance-history row of the step
+    double p_index;     // p
+    double r0[2];       // vcore's r0 at p and p + 1
+    double T[2];        // fastout.Time at p and p + 1
+    double wind[2][3];  // wind.infty at p and p + 1
+};
+
+__global__ void k_set_step(StepDev *__restrict__ sd, StepDev v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *sd = v;
+}
+
 // wake ages 0 and 1 of time index p0+1: pos.quarter and pos.trail
 // (initials.m:87-88, kinematics.m:49-50)
-__global__ void k_nodes01(Geom g, int p0, int A, double *__restrict__ X) {
+// (sd: table index sd->p0 + dp instead of p0)
+__global__ void k_nodes01(Geom g, int p0, int A, double *__restrict__ X,
+                          const StepDev *__restrict__ sd, int dp) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= g.nst * g.nb) return;
+    if (sd) p0 = sd->p0 + dp;
     const int m = i % g.nst, b = i / g.nst;
     const int64_t N = (int64_t)g.nst * A * g.nb;
     const int64_t j0 = m + (int64_t)b * g.nst * A;
@@ -202,13 +223,19 @@ __global__ void k_kelvin_shed(int nst, int nb, const double *__restrict__ gs_old
 }
 
 // vcore.m:30-51 + FilamentLength.m + filamentmod.m:27-41 for one time index
+// (sd: r0 and T of step slot `slot`, 0 for p, 1 for p + 1)
 __global__ void k_vcore(int nst, int nb, int A, const double *__restrict__ X,
                         const double *__restrict__ gs, const double *__restrict__ gt, double r0,
                         double T, int fmod, const double *__restrict__ ls_prev,
                         const double *__restrict__ lt_prev, int A_prev,
                         double *__restrict__ ls, double *__restrict__ lt,
-                        double *__restrict__ rs, double *__restrict__ rt) {
+                        double *__restrict__ rs, double *__restrict__ rt,
+                        const StepDev *__restrict__ sd, int slot) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (sd) {
+        r0 = sd->r0[slot];
+        T = sd->T[slot];
+    }
     const int ns = nst - 1;
     const int64_t N = (int64_t)nst * A * nb;
     const int64_t n_s = (int64_t)ns * A * nb;
@@ -400,9 +427,15 @@ __global__ void k_max_err(int64_t n, const double *__restrict__ a, const double 
 }
 
 __global__ void k_add_uinf(int64_t n, double ux, double uy, double uz,
-                           const double *__restrict__ u, double *__restrict__ v) {
+                           const double *__restrict__ u, double *__restrict__ v,
+                           const StepDev *__restrict__ sd, int slot) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (sd) {
+        ux = sd->wind[slot][0];
+        uy = sd->wind[slot][1];
+        uz = sd->wind[slot][2];
+    }
     v[i] = ux + u[i];
     v[n + i] = uy + u[n + i];
     v[2 * n + i] = uz + u[2 * n + i];
@@ -465,10 +498,12 @@ __global__ void k_advect(int nst, int nb, int A_old, int A_new, double dt,
     }
 }
 
-__global__ void k_ll_points(Geom g, int p0, double *__restrict__ P) {
+__global__ void k_ll_points(Geom g, int p0, double *__restrict__ P,
+                            const StepDev *__restrict__ sd) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = g.ns * g.nb;
     if (i >= n) return;
+    if (sd) p0 = sd->p0;
     const LL L = ll_frame(g, i % g.ns, i / g.ns, p0);
     P[i] = L.px;
     P[n + i] = L.py;
@@ -546,7 +581,15 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double
                                                   double *__restrict__ cd_out,
                                                   double *__restrict__ aoa_out,
                                                   double *__restrict__ hist_row, double p_index,
-                                                  int *__restrict__ flag) {
+                                                  int *__restrict__ flag,
+                                                  const StepDev *__restrict__ sd) {
+    // sd: the table index, the history row (hist_row is then the history's
+    // start) and p from the step's device values
+    if (sd) {
+        p0 = sd->p0;
+        hist_row += 5 * (int64_t)sd->row;
+        p_index = sd->p_index;
+    }
     const int nst = g.nst, ns = g.ns, nb = g.nb;
     const int NLL = ns * nb;
     const int per = 2 * ns + nst;
@@ -831,6 +874,13 @@ struct winds_march {
     DBuf gs[2], gt[2], gs1, gt1, rs, rt, rs1, rt1, ls[3], lt[3];
     int ls_A[3] = {0, 0, 0};
     DBuf segs, ws, llp, frozen, K, kjout, perf, flag;
+    // per-step device values (StepDev) and the recorded steps of a steady
+    // march, one graph per phase of the buffer rotations
+    DBuf stepd;
+    bool in_step = false;
+    bool use_graphs = false;
+    int steady_eager = 0;
+    std::map<int, hipGraphExec_t> graphs;
     // far-wake surrogate (winds_march_set_surrogate); net is not owned
     winds_mlp *net = nullptr;
     winds_march_surr sc{};
@@ -947,10 +997,11 @@ int vcore(winds_march *m, int p, const double *X, int A, const double *gsp, cons
     const double T = m->h_time[p - 1];  // vcore.m:34-37: fastout.Time(time)
     const int fmod = p > 3 ? 1 : 0;
     const int64_t n = segs_of(m, A);
+    const StepDev *sd = m->in_step ? (const StepDev *)m->stepd.p : nullptr;
     k_vcore<<<blocks(n), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, X, gsp, gtp, r0_of(m, p), T, fmod,
                                           m->ls[lprev].d(), m->lt[lprev].d(),
                                           std::max(2, m->ls_A[lprev]), m->ls[lout].d(),
-                                          m->lt[lout].d(), rsp, rtp);
+                                          m->lt[lout].d(), rsp, rtp, sd, p == m->p ? 0 : 1);
     WM_HIP(hipGetLastError());
     m->ls_A[lout] = A;
     return WINDS_OK;
@@ -958,7 +1009,9 @@ int vcore(winds_march *m, int p, const double *X, int A, const double *gsp, cons
 
 int advect(winds_march *m, const double *Xo, int A_old, const double *vo, const double *vpred,
            int ab, int p_next, double *Xn, int A_new) {
-    k_nodes01<<<blocks((int64_t)m->g.nst * m->g.nb), 256, 0, m->st>>>(m->g, p_next - 1, A_new, Xn);
+    // in a step, p_next - 1 = p: the step's table index + 1
+    k_nodes01<<<blocks((int64_t)m->g.nst * m->g.nb), 256, 0, m->st>>>(
+        m->g, p_next - 1, A_new, Xn, m->in_step ? (const StepDev *)m->stepd.p : nullptr, 1);
     WM_HIP(hipGetLastError());
     const int64_t n = (int64_t)m->g.nst * std::max(0, A_new - 2) * m->g.nb;
     if (n > 0) {
@@ -1059,7 +1112,7 @@ int surr_check(winds_march *m, const double *X, int A, const double *gsp, const 
     const int NLL = m->g.ns * m->g.nb;
     k_gather_near<<<blocks(Nt), 256, 0, m->st>>>(m->g.nst, m->g.nb, A, St, X, m->tgt.d());
     WM_HIP(hipGetLastError());
-    k_ll_points<<<blocks(NLL), 256, 0, m->st>>>(m->g, t - 1, m->llp.d());
+    k_ll_points<<<blocks(NLL), 256, 0, m->st>>>(m->g, t - 1, m->llp.d(), nullptr);
     WM_HIP(hipGetLastError());
     m->stats[0] += 1;
     if (m->sc.mode != WINDS_SURR_MLP) return WINDS_OK;
@@ -1156,7 +1209,9 @@ int wake_velocity(winds_march *m, const double *X, int A, const double *gsp, con
 int add_wind(winds_march *m, int p, int64_t N, const double *u, double *v) {
     const int nt = m->cfg.n_times;
     const double *w = m->h_wind.data();
-    k_add_uinf<<<blocks(N), 256, 0, m->st>>>(N, w[p - 1], w[nt + p - 1], w[2 * nt + p - 1], u, v);
+    k_add_uinf<<<blocks(N), 256, 0, m->st>>>(N, w[p - 1], w[nt + p - 1], w[2 * nt + p - 1], u, v,
+                                             m->in_step ? (const StepDev *)m->stepd.p : nullptr,
+                                             p == m->p ? 0 : 1);
     WM_HIP(hipGetLastError());
     return WINDS_OK;
 }
@@ -1165,7 +1220,7 @@ int add_wind(winds_march *m, int p, int64_t N, const double *u, double *v) {
 int setup(winds_march *m) {
     const int A = 2;
     double *X = m->nodes[0].d();
-    k_nodes01<<<blocks((int64_t)m->g.nst * m->g.nb), 256, 0, m->st>>>(m->g, 0, A, X);
+    k_nodes01<<<blocks((int64_t)m->g.nst * m->g.nb), 256, 0, m->st>>>(m->g, 0, A, X, nullptr, 0);
     WM_HIP(hipGetLastError());
     // the bound circulation as a one-age shed array feeds the Kelvin step
     k_init_bound<<<blocks((int64_t)m->g.ns * m->g.nb), 256, 0, m->st>>>(
@@ -1195,14 +1250,51 @@ int setup(winds_march *m) {
     return WINDS_OK;
 }
 
-// WInDS.m:127-183, one time index p
-int step_once(winds_march *m) {
+// Per-step values of time index p (and p + 1) into device memory
+// (StepDev), enqueued before the step's other launches.
+int set_step(winds_march *m) {
+    const int p = m->p;
+    const int nt = m->cfg.n_times;
+    const double *w = m->h_wind.data();
+    StepDev v{};
+    v.p0 = p - 1;
+    v.row = m->nsteps_done;
+    v.p_index = (double)p;
+    for (int k = 0; k < 2; ++k) {
+        const int q = p + k;  // q <= n_times (step_check)
+        v.r0[k] = r0_of(m, q);
+        v.T[k] = m->h_time[q - 1];
+        v.wind[k][0] = w[q - 1];
+        v.wind[k][1] = w[nt + q - 1];
+        v.wind[k][2] = w[2 * nt + q - 1];
+    }
+    k_set_step<<<1, 64, 0, m->st>>>((StepDev *)m->stepd.p, v);
+    WM_HIP(hipGetLastError());
+    return WINDS_OK;
+}
+
+int step_check(const winds_march *m) {
     const int p = m->p;
     if (p + 1 > m->cfg.n_times)
         return set_error(WINDS_E_DIMS, "the kinematic tables end at time index " +
                                            std::to_string(m->cfg.n_times) +
                                            "; stepping time index " + std::to_string(p) +
                                            " needs the next one");
+    if (m->nsteps_done >= m->cfg.n_times)
+        return set_error(WINDS_E_NOMEM, "performance history is full");
+    return WINDS_OK;
+}
+
+// What a step changes on the host besides its launches.
+struct StepPlan {
+    int lnow, lpred, A_new;
+    bool pcc;  // the corrector ran (vcore of p + 1 into lpred)
+};
+
+// WInDS.m:127-183, the launches of time index p (after set_step), with the
+// host bookkeeping they imply except the rotation step_finish does
+int step_body(winds_march *m, StepPlan &sp) {
+    const int p = m->p;
     const int t = p;
     const int A = m->A;
     const int nc = m->cur, nn = 1 - m->cur;        // nodes: time p / time p+1
@@ -1212,6 +1304,9 @@ int step_once(winds_march *m) {
     int rc = kelvin(m, m->gs[gp].d(), m->A_gprev, m->gt[gp].d(), A, gs, gt);
     if (rc) return rc;
     const int lprev = m->lcur, lnow = (m->lcur + 1) % 3, lpred = (m->lcur + 2) % 3;
+    sp.lnow = lnow;
+    sp.lpred = lpred;
+    sp.pcc = false;
     rc = vcore(m, p, X, A, gs, gt, lprev, lnow, m->rs.d(), m->rt.d());
     if (rc) return rc;
     const int64_t N = nodes_of(m, A);
@@ -1224,6 +1319,7 @@ int step_once(winds_march *m) {
     rc = add_wind(m, p, N, m->uind.d(), m->vel.d());
     if (rc) return rc;
     const int A_new = std::min(A + 1, m->cfg.max_ages);
+    sp.A_new = A_new;
     double *Xn = m->nodes[nn].d();
     const int integ = m->cfg.integrator;
     if (integ == WINDS_INTEG_AB4 && p >= 4) {
@@ -1235,6 +1331,7 @@ int step_once(winds_march *m) {
         if (!rc && integ == WINDS_INTEG_PCC && p > 2) {
             // pcc.m:40-75: Kelvin and vcore for p+1 at the predicted nodes,
             // induced velocity there, trapezoidal corrector
+            sp.pcc = true;
             rc = kelvin(m, gs, A, gt, A_new, m->gs1.d(), m->gt1.d());
             if (!rc)
                 rc = vcore(m, p + 1, Xn, A_new, m->gs1.d(), m->gt1.d(), lnow, lpred, m->rs1.d(),
@@ -1250,8 +1347,9 @@ int step_once(winds_march *m) {
     }
     if (rc) return rc;
     // Kutta-Joukowski at time p on the wake of time p (KuttaJoukowski.m:215-219)
+    const StepDev *sd = (const StepDev *)m->stepd.p;
     const int NLL = m->g.ns * m->g.nb;
-    k_ll_points<<<blocks(NLL), 256, 0, m->st>>>(m->g, p - 1, m->llp.d());
+    k_ll_points<<<blocks(NLL), 256, 0, m->st>>>(m->g, p - 1, m->llp.d(), sd);
     WM_HIP(hipGetLastError());
     if (how == kSurrogate) {
         rc = induced(m, X, A, gs, gt, m->rs.d(), m->rt.d(), m->llp.d(), NLL, m->frozen.d(), m->ll,
@@ -1269,12 +1367,11 @@ int step_once(winds_march *m) {
         m->g, A, X, m->rs.d(), m->rt.d(), m->llp.d(), m->ll.semantics,
         m->ll.core == 0 ? 0 : m->ll.vat_n, m->ll.d, m->ll.co, m->K.d());
     WM_HIP(hipGetLastError());
-    const int row = m->nsteps_done;
-    if (row >= m->cfg.n_times) return set_error(WINDS_E_NOMEM, "performance history is full");
     double *kj = m->kjout.d();
+    // the history row and p come from sd
     k_kj<<<1, kKjThreads, 0, m->st>>>(m->g, p - 1, A, m->cfg.relax, m->cfg.tol, m->cfg.maxiter,
                                        m->frozen.d(), m->K.d(), gs, gt, kj, kj + NLL, kj + 2 * NLL,
-                                       m->perf.d() + 5 * (int64_t)row, (double)p, (int *)m->flag.p);
+                                       m->perf.d(), (double)p, (int *)m->flag.p, sd);
     WM_HIP(hipGetLastError());
     rc = push_hist(m, m->vel.d(), A);
     if (rc) return rc;
@@ -1283,14 +1380,90 @@ int step_once(winds_march *m) {
             m->stats[(m->sc.mode == WINDS_SURR_MLP && !m->fallback) ? 2 : 3] += 1;
         m->since_check++;
     }
-    m->nsteps_done++;
-    m->lcur = lnow;
-    m->gprev = gc;
-    m->A_gprev = A;
-    m->cur = nn;
-    m->A = A_new;
-    m->p = p + 1;
     return WINDS_OK;
+}
+
+// The buffer rotation that ends a step.
+void step_finish(winds_march *m, const StepPlan &sp) {
+    m->nsteps_done++;
+    m->lcur = sp.lnow;
+    m->gprev = 1 - m->gprev;
+    m->A_gprev = m->A;
+    m->cur = 1 - m->cur;
+    m->A = sp.A_new;
+    m->p = m->p + 1;
+}
+
+int step_once(winds_march *m) {
+    int rc = step_check(m);
+    if (!rc) rc = set_step(m);
+    if (rc) return rc;
+    StepPlan sp{};
+    m->in_step = true;
+    rc = step_body(m, sp);
+    m->in_step = false;
+    if (rc) return rc;
+    step_finish(m, sp);
+    return WINDS_OK;
+}
+
+void drop_graphs(winds_march *m) {
+    for (auto &kv : m->graphs) (void)hipGraphExecDestroy(kv.second);
+    m->graphs.clear();
+    m->steady_eager = 0;
+}
+
+// A step of a steady direct march (the wake at max_ages, no surrogate):
+// every launch is the same as in the steps six before and after it (the
+// node, circulation, length and history buffers rotate with periods 2, 2, 3
+// and 3), and everything that varies is read from StepDev, so each phase
+// of the rotation is recorded once into a graph and replayed.
+int step_graph(winds_march *m) {
+    int rc = step_check(m);
+    if (!rc) rc = set_step(m);
+    if (rc) return rc;
+    const int key = m->cur + 2 * m->gprev + 4 * m->lcur + 12 * (m->n_hist % 3);
+    StepPlan sp{};
+    auto it = m->graphs.find(key);
+    if (it == m->graphs.end()) {
+        WM_HIP(hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal));
+        m->in_step = true;
+        rc = step_body(m, sp);
+        m->in_step = false;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(m->st, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess)
+            return set_error(WINDS_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiateWithFlags(&ex, g, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess)
+            return set_error(WINDS_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        it = m->graphs.emplace(key, ex).first;
+    } else {
+        // the host bookkeeping of the recorded launches (vcore's lengths,
+        // the velocity history)
+        const int A = m->A;
+        sp.lnow = (m->lcur + 1) % 3;
+        sp.lpred = (m->lcur + 2) % 3;
+        sp.A_new = std::min(A + 1, m->cfg.max_ages);
+        sp.pcc = m->cfg.integrator == WINDS_INTEG_PCC && m->p > 2;
+        m->ls_A[sp.lnow] = A;
+        if (sp.pcc) m->ls_A[sp.lpred] = sp.A_new;
+        m->hist_A[m->n_hist % 3] = A;
+        m->n_hist++;
+    }
+    WM_HIP(hipGraphLaunch(it->second, m->st));
+    step_finish(m, sp);
+    return WINDS_OK;
+}
+
+bool steady_direct(const winds_march *m) {
+    return m->use_graphs && !m->surr && m->A == m->cfg.max_ages && m->A_gprev == m->A;
 }
 
 }  // namespace
@@ -1300,6 +1473,7 @@ extern "C" {
 void winds_march_destroy(winds_march *m) {
     if (!m) return;
     if (m->st) (void)hipStreamSynchronize(m->st);
+    drop_graphs(m);
     if (m->st) (void)hipStreamDestroy(m->st);
     delete m;
 }
@@ -1455,6 +1629,7 @@ int winds_march_create(const winds_march_cfg *cfg, winds_march **out) {
     WMC(m->kjout.alloc((size_t)3 * NLL * sizeof(double)));
     WMC(m->perf.alloc((size_t)5 * nt * sizeof(double)));
     WMC(m->flag.alloc(256));
+    WMC(m->stepd.alloc(sizeof(StepDev)));
     WMC(hipMemsetAsync(m->flag.p, 0, 256, m->st));
     WMC(hipMemsetAsync(m->gt1.p, 0, ST, m->st));
     for (int i = 0; i < 3; ++i) {
@@ -1472,9 +1647,29 @@ int winds_march_step(winds_march *m, int32_t nsteps) {
     winds::clear_error();
     if (!m) return set_error(WINDS_E_ARG, "null march");
     for (int i = 0; i < nsteps; ++i) {
-        int rc = step_once(m);
+        int rc;
+        if (!steady_direct(m)) {
+            m->steady_eager = 0;
+            rc = step_once(m);
+        } else if (m->steady_eager < 2) {
+            // steady steps run eagerly twice first (the workspace is then
+            // sized for them) before any is recorded
+            rc = step_once(m);
+            m->steady_eager++;
+        } else {
+            rc = step_graph(m);
+        }
         if (rc) return rc;
     }
+    return WINDS_OK;
+}
+
+int winds_march_set_graphs(winds_march *m, int32_t on) {
+    winds::clear_error();
+    if (!m) return set_error(WINDS_E_ARG, "null march");
+    WM_HIP(hipStreamSynchronize(m->st));
+    drop_graphs(m);
+    m->use_graphs = on != 0;
     return WINDS_OK;
 }
 
@@ -1537,6 +1732,7 @@ int winds_march_set_state(winds_march *m, int32_t p, int32_t ages, const double 
     if (ages < 3 || ages > m->cfg.max_ages || p < 4 || p >= m->cfg.n_times)
         return set_error(WINDS_E_DIMS, "bad ages/p");
     WM_HIP(hipStreamSynchronize(m->st));
+    drop_graphs(m);
     const int nst = m->g.nst, ns = m->g.ns, nb = m->g.nb, A = ages;
     const int64_t N = nodes_of(m, A);
     std::vector<double> h((size_t)3 * N);
@@ -1563,7 +1759,7 @@ int winds_march_set_state(winds_march *m, int32_t p, int32_t ages, const double 
     k_vcore<<<blocks(ns_), 256, 0, m->st>>>(nst, nb, A, m->nodes[m->cur].d(), m->gs[m->gprev].d(),
                                             m->gt[m->gprev].d(), r0_of(m, p), 0.0, 0, m->ls[0].d(),
                                             m->lt[0].d(), A, m->ls[m->lcur].d(), m->lt[m->lcur].d(),
-                                            m->rs.d(), m->rt.d());
+                                            m->rs.d(), m->rt.d(), nullptr, 0);
     WM_HIP(hipGetLastError());
     m->ls_A[m->lcur] = A;
     m->A_ref = 0;
@@ -1600,6 +1796,7 @@ int winds_march_set_surrogate(winds_march *m, winds_mlp *net, const winds_march_
     winds::clear_error();
     if (!m) return set_error(WINDS_E_ARG, "null march");
     WM_HIP(hipStreamSynchronize(m->st));
+    drop_graphs(m);
     if (!s) {
         m->surr = false;
         m->net = nullptr;

@@ -66,8 +66,9 @@ def _bind(L):
     L.winds_march_uind_dev.restype = C.c_void_p
     L.winds_march_set_surrogate.argtypes = [P, P, C.POINTER(MarchSurr)]
     L.winds_march_surr_stats.argtypes = [P, D]
+    L.winds_march_set_graphs.argtypes = [P, C.c_int32]
     for f in ("create", "step", "sync", "info", "get_state", "set_state", "get_perf", "get_kj",
-              "set_surrogate", "surr_stats"):
+              "set_surrogate", "surr_stats", "set_graphs"):
         getattr(L, "winds_march_" + f).restype = C.c_int
     L._winds_march_bound = True
     return L
@@ -128,6 +129,11 @@ class March:
 
     def sync(self):
         _lib.check(self.L.winds_march_sync(self._m))
+
+    def set_graphs(self, on: bool = True):
+        """Replay recorded HIP graphs of the steps once the march is steady
+        (wake at max_ages, no surrogate); the results do not change."""
+        _lib.check(self.L.winds_march_set_graphs(self._m, 1 if on else 0))
 
     def info(self) -> dict:
         a = (C.c_int64 * 6)()
