@@ -115,6 +115,16 @@ def test_full_length_geometry_matches_oracle(need_gpu):
     compare(m, om, "3 x 31, 5 deg")
 
 
+def test_long_lifting_line_matches_oracle(need_gpu):
+    """3 blades x 41 nodes (120 lifting-line points): the Kutta-Joukowski
+    matrix in the chord frames (2 x 120 x 120 doubles) exceeds a workgroup's
+    LDS, so the kernel forms its terms from device memory at every
+    iteration, the path the 3 x 31 and NREL rotors do not take."""
+    tb = cases.tables(cases.synthetic_rotor(nb=3, nst=41, n_times=20))
+    m, om = run_pair(tb, 8, 12, "mex", 1e-4, "pcc")
+    compare(m, om, "3 x 41")
+
+
 def test_zero_lift_wake_convects_with_the_free_stream(need_gpu):
     """No lift: no circulation and no induction, so a node of age a >= 1
     sits where the trailing edge was a-1 steps ago, moved by the free

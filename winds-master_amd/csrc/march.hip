@@ -557,12 +557,70 @@ __global__ void k_influence(Geom g, int A, const double *__restrict__ X,
     K[i * 3 + 2] = uz;
 }
 
+// The changing filaments' velocity at the lifting line as a function of the
+// bound circulation alone.  Per blade b the changing circulation is, from
+// the bound values G[m] (KuttaJoukowski.m:285-288): shed age 0 G[m], shed
+// age 1 c1[m] - G[m] (c1 the span cumsum of trailing age 1, fixed during
+// the iteration), trailing age 0 G[m] - G[m-1] (G[-1] = G[ns] = 0).  So
+// u_i = u_frozen,i + sum_m K_i(shed 1, m) c1[m] + sum_{b,m} M_i(b,m) G[b][m]
+// with M_i(b,m) = K_i(shed 0, m) - K_i(shed 1, m) + K_i(trail 0, m)
+// - K_i(trail 0, m+1).  M [NLL][NLL][3] is a third of K, and the iteration
+// reads only it.
+// M is stored [j][component][i], so that the points i of one bound value j
+// and component are adjacent.
+__global__ void k_kj_matrix(Geom g, const double *__restrict__ K, double *__restrict__ M) {
+    const int ns = g.ns, nst = g.nst, nb = g.nb;
+    const int per = 2 * ns + nst, NCH = per * nb, NLL = ns * nb;
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)NLL * NLL) return;
+    const int j = (int)(k / NLL), i = (int)(k % NLL);
+    const int b = j / ns, m = j % ns;
+    const double *Ki = K + (int64_t)i * NCH * 3;
+    const int s0 = b * per + m, s1 = b * per + ns + m, t0 = b * per + 2 * ns + m;
+    for (int c = 0; c < 3; ++c)
+        M[((int64_t)j * 3 + c) * NLL + i] =
+            ((Ki[3 * s0 + c] - Ki[3 * s1 + c]) + Ki[3 * t0 + c]) - Ki[3 * (t0 + 1) + c];
+}
+
 constexpr int kKjThreads = 1024;
 constexpr int kKjWaves = kKjThreads / 64;
 constexpr int kMaxLL = 512;      // lifting-line points handled by one workgroup
 constexpr int kMaxNodesBlade = 256;
 constexpr int kMaxAlpha = 1024;  // polar table entries (all tables) held in LDS by the KJ kernel
 constexpr int kMaxFoils = 64;    // airfoil tables
+
+// The KJ kernel's LDS, sized to the rotor at launch (offsets in doubles):
+// bound circulation, span cumsum of trailing age 1, the frozen velocity,
+// the per-point constants, partial sums, the polars and, when it fits,
+// the changing filaments' matrix in each point's chord frame.
+struct KjLds {
+    int kparts, g, c1, fz, ll, part, al, tcl, tcd, mp, total;
+};
+__host__ __device__ inline KjLds kj_lds(int NLL, int n_alpha, bool with_m) {
+    KjLds L;
+    L.kparts = kKjThreads / ((NLL + 63) & ~63);
+    int o = 0;
+    L.g = o;
+    o += NLL;
+    L.c1 = o;
+    o += NLL;
+    L.fz = o;
+    o += 3 * NLL;
+    L.ll = o;
+    o += 10 * NLL;
+    L.part = o;
+    o += 2 * L.kparts * NLL;
+    L.al = o;
+    o += n_alpha;
+    L.tcl = o;
+    o += n_alpha;
+    L.tcd = o;
+    o += n_alpha;
+    L.mp = o;
+    if (with_m) o += 2 * NLL * NLL;
+    L.total = o;
+    return L;
+}
 
 // Kutta-Joukowski fixed point (KuttaJoukowski.m:42-49 with kj() :204-296)
 // for time index p0+1, in FP64, in one workgroup, followed by
@@ -576,13 +634,14 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double
                                                   double tol, int maxiter,
                                                   const double *__restrict__ frozen,
                                                   const double *__restrict__ K,
+                                                  const double *__restrict__ M,
                                                   double *__restrict__ gs, double *__restrict__ gt,
                                                   double *__restrict__ cl_out,
                                                   double *__restrict__ cd_out,
                                                   double *__restrict__ aoa_out,
                                                   double *__restrict__ hist_row, double p_index,
                                                   int *__restrict__ flag,
-                                                  const StepDev *__restrict__ sd) {
+                                                  const StepDev *__restrict__ sd, int m_lds) {
     // sd: the table index, the history row (hist_row is then the history's
     // start) and p from the step's device values
     if (sd) {
@@ -594,16 +653,18 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double
     const int NLL = ns * nb;
     const int per = 2 * ns + nst;
     const int NCH = per * nb;
-    __shared__ double s_g[kMaxLL];      // bound circulation
-    __shared__ double s_gam[3 * kMaxLL + 4 * kMaxNodesBlade];  // changing circulation by s
-    __shared__ double s_c1[kMaxLL];     // span cumsum of trailing age 1
-    __shared__ double s_u[3 * kMaxLL];  // induced velocity at the lifting line
+    extern __shared__ double s_dyn[];
+    const KjLds L = kj_lds(NLL, g.n_alpha, m_lds != 0);
+    double *s_g = s_dyn + L.g;        // bound circulation
+    double *s_c1 = s_dyn + L.c1;      // span cumsum of trailing age 1
+    double *s_part = s_dyn + L.part;  // partial sums of the changing filaments
+    double *s_al = s_dyn + L.al, *s_tcl = s_dyn + L.tcl, *s_tcd = s_dyn + L.tcd;  // the polars
+    double *s_fz = s_dyn + L.fz;  // the frozen filaments' velocity at the points
+    double *s_ll = s_dyn + L.ll;  // per point: chord frame, inflow, |inflow|, chord
+    double *s_mp = s_dyn + L.mp;  // M in the chord frames (m_lds)
     __shared__ double s_red[kKjWaves];
     __shared__ double s_red2[kKjWaves];
-    __shared__ double s_al[kMaxAlpha], s_tcl[kMaxAlpha], s_tcd[kMaxAlpha];  // the polars
     __shared__ int s_foff[kMaxFoils], s_flen[kMaxFoils];
-    __shared__ double s_fz[3 * kMaxLL];   // the frozen filaments' velocity at the points
-    __shared__ double s_ll[10 * kMaxLL];  // per point: chord frame, inflow, |inflow|, chord
     __shared__ int s_nan;
     const int tid = threadIdx.x;
     for (int i = tid; i < NLL; i += kKjThreads) {
@@ -658,65 +719,81 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double
         }
     }
     if (tid == 0) s_nan = 0;
+    // split of the bound values for the changing filaments' sum: kparts
+    // ranges, each over NLL points rounded up to a wave
+    const int nllw = (NLL + 63) & ~63;
+    const int kparts = L.kparts;
+    const int kpart = tid / nllw, ki = tid % nllw;
+    __syncthreads();
+    // the part that does not change during the iteration: the frozen
+    // filaments and shed age 1's c1 share (shed age 1 = c1 - G), in the
+    // chord frame of the point (the iteration needs only the two in-plane
+    // components)
+    double rx0 = 0.0, ry0 = 0.0;
+    if (tid < NLL) {
+        const double *Ki = K + (int64_t)tid * NCH * 3;
+        double cx = 0.0, cy = 0.0, cz = 0.0;
+        for (int b = 0; b < nb; ++b)
+            for (int m = 0; m < ns; ++m) {
+                const double c1 = s_c1[b * ns + m];
+                const int sj = b * per + ns + m;
+                cx += Ki[3 * sj] * c1;
+                cy += Ki[3 * sj + 1] * c1;
+                cz += Ki[3 * sj + 2] * c1;
+            }
+        const double ux = s_fz[tid] + cx, uy = s_fz[NLL + tid] + cy, uz = s_fz[2 * NLL + tid] + cz;
+        const double *q = s_ll + 10 * tid;
+        rx0 = q[0] * ux + q[1] * uy + q[2] * uz;
+        ry0 = q[3] * ux + q[4] * uy + q[5] * uz;
+    }
+    // The changing filaments' share from the bound values: they split into
+    // kparts ranges, one thread per (range, point ki); each term is M's
+    // column for (ki, j) turned into ki's chord frame.  With m_lds the terms
+    // are formed once into LDS; otherwise they are formed from M at every
+    // iteration, with the same arithmetic.
+    const int jlo = (int)((int64_t)kpart * NLL / kparts);
+    const int jhi = (int)((int64_t)(kpart + 1) * NLL / kparts);
+    const bool active = kpart < kparts && ki < NLL;
+    double bxk[3] = {0.0, 0.0, 0.0}, byk[3] = {0.0, 0.0, 0.0};
+    if (active) {
+        const double *q = s_ll + 10 * ki;
+        bxk[0] = q[0];
+        bxk[1] = q[1];
+        bxk[2] = q[2];
+        byk[0] = q[3];
+        byk[1] = q[4];
+        byk[2] = q[5];
+        if (m_lds)
+            for (int jj = jlo; jj < jhi; ++jj) {
+                const double *Mj = M + (int64_t)jj * 3 * NLL + ki;
+                const double a = Mj[0], b = Mj[NLL], c = Mj[2 * NLL];
+                s_mp[2 * jj * NLL + ki] = bxk[0] * a + bxk[1] * b + bxk[2] * c;
+                s_mp[(2 * jj + 1) * NLL + ki] = byk[0] * a + byk[1] * b + byk[2] * c;
+            }
+    }
     __syncthreads();
     double dgmax = 1.0;
     int j = 0;
     while (dgmax > tol && j < maxiter) {
-        // circulation of the changing filaments from the current bound values
-        if (tid < nb) {
-            const int b = tid;
-            double *sg = s_gam + b * per;
-            double c0 = 0.0;
-            for (int m = 0; m < nst; ++m) {
-                const double hi = m < ns ? s_g[b * ns + m] : 0.0;
-                const double lo = m > 0 ? s_g[b * ns + m - 1] : 0.0;
-                const double tr0 = hi - lo;
-                sg[2 * ns + m] = tr0;
-                if (m < ns) {
-                    c0 += tr0;
-                    sg[m] = s_g[b * ns + m];
-                    sg[ns + m] = s_c1[b * ns + m] - c0;
+        if (active) {
+            double px = 0.0, py = 0.0;
+            if (m_lds) {
+                for (int jj = jlo; jj < jhi; ++jj) {
+                    const double gj = s_g[jj];
+                    px += s_mp[2 * jj * NLL + ki] * gj;
+                    py += s_mp[(2 * jj + 1) * NLL + ki] * gj;
+                }
+            } else {
+                for (int jj = jlo; jj < jhi; ++jj) {
+                    const double *Mj = M + (int64_t)jj * 3 * NLL + ki;
+                    const double a = Mj[0], b = Mj[NLL], c = Mj[2 * NLL];
+                    const double gj = s_g[jj];
+                    px += (bxk[0] * a + bxk[1] * b + bxk[2] * c) * gj;
+                    py += (byk[0] * a + byk[1] * b + byk[2] * c) * gj;
                 }
             }
-        }
-        __syncthreads();
-        // the changing filaments' velocity at lifting-line point i: one wave
-        // per point, lanes striding over the filaments, then a wave sum
-        {
-            const int wave = tid >> 6, lane = tid & 63;
-            for (int i = wave; i < NLL; i += kKjWaves) {
-                const double *Ki = K + (int64_t)i * NCH * 3;
-                double ux = 0.0, uy = 0.0, uz = 0.0;
-                // four filaments per lane per pass, their loads issued together
-                for (int s0 = lane; s0 < NCH; s0 += 4 * 64) {
-                    double kv[4][3], gv[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int s = s0 + 64 * q;
-                        const int sc = s < NCH ? s : 0;
-                        gv[q] = s < NCH ? s_gam[sc] : 0.0;
-                        kv[q][0] = Ki[3 * sc];
-                        kv[q][1] = Ki[3 * sc + 1];
-                        kv[q][2] = Ki[3 * sc + 2];
-                    }
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        ux += gv[q] * kv[q][0];
-                        uy += gv[q] * kv[q][1];
-                        uz += gv[q] * kv[q][2];
-                    }
-                }
-                for (int off = 32; off > 0; off >>= 1) {
-                    ux += __shfl_down(ux, off);
-                    uy += __shfl_down(uy, off);
-                    uz += __shfl_down(uz, off);
-                }
-                if (lane == 0) {
-                    s_u[i] = s_fz[i] + ux;
-                    s_u[NLL + i] = s_fz[NLL + i] + uy;
-                    s_u[2 * NLL + i] = s_fz[2 * NLL + i] + uz;
-                }
-            }
+            s_part[kpart * NLL + ki] = px;
+            s_part[(kparts + kpart) * NLL + ki] = py;
         }
         __syncthreads();
         double gnew = 0.0;
@@ -724,9 +801,11 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double
         if (tid < NLL) {
             const int i = tid;
             const double *q = s_ll + 10 * i;
-            const double ux = s_u[i], uy = s_u[NLL + i], uz = s_u[2 * NLL + i];
-            const double rx = q[0] * ux + q[1] * uy + q[2] * uz;
-            const double ry = q[3] * ux + q[4] * uy + q[5] * uz;
+            double rx = rx0, ry = ry0;
+            for (int r = 0; r < kparts; ++r) {
+                rx += s_part[r * NLL + i];
+                ry += s_part[(kparts + r) * NLL + i];
+            }
             const double tu = q[6] + rx, tv = q[7] + ry;
             const double Vinf = q[8], chord = q[9];
             const double aoa = atan2(-tv, tu) * (180.0 / kPi);
@@ -871,9 +950,12 @@ struct winds_march {
     DBuf geom, nodes[2], vel, velp, uind, hist[3];
     int hist_A[3] = {0, 0, 0};
     int n_hist = 0;
+    // the KJ kernel's LDS bytes, and whether its matrix is held there
+    size_t kj_lds_bytes = 0;
+    int kj_m_lds = 0;
     DBuf gs[2], gt[2], gs1, gt1, rs, rt, rs1, rt1, ls[3], lt[3];
     int ls_A[3] = {0, 0, 0};
-    DBuf segs, ws, llp, frozen, K, kjout, perf, flag;
+    DBuf segs, ws, llp, frozen, K, Mkj, kjout, perf, flag;
     // per-step device values (StepDev) and the recorded steps of a steady
     // march, one graph per phase of the buffer rotations
     DBuf stepd;
@@ -1293,6 +1375,26 @@ struct StepPlan {
 
 // WInDS.m:127-183, the launches of time index p (after set_step), with the
 // host bookkeeping they imply except the rotation step_finish does
+// LDS for the KJ kernel: the matrix goes in when the workgroup limit allows
+// it, else the kernel reads it from device memory at every iteration.
+int kj_lds_plan(winds_march *m, int NLL, int n_alpha) {
+    int dev = 0, lim = 0;
+    WM_HIP(hipGetDevice(&dev));
+    WM_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    hipFuncAttributes fa;
+    WM_HIP(hipFuncGetAttributes(&fa, (const void *)k_kj));
+    const size_t avail = (size_t)lim > fa.sharedSizeBytes ? (size_t)lim - fa.sharedSizeBytes : 0;
+    const size_t with_m = (size_t)kj_lds(NLL, n_alpha, true).total * sizeof(double);
+    const size_t without = (size_t)kj_lds(NLL, n_alpha, false).total * sizeof(double);
+    m->kj_m_lds = with_m <= avail;
+    m->kj_lds_bytes = m->kj_m_lds ? with_m : without;
+    if (m->kj_lds_bytes > avail)
+        return set_error(WINDS_E_ARG, "march: the lifting line needs more LDS than a workgroup has");
+    WM_HIP(hipFuncSetAttribute((const void *)k_kj, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)m->kj_lds_bytes));
+    return 0;
+}
+
 int step_body(winds_march *m, StepPlan &sp) {
     const int p = m->p;
     const int t = p;
@@ -1369,9 +1471,12 @@ int step_body(winds_march *m, StepPlan &sp) {
     WM_HIP(hipGetLastError());
     double *kj = m->kjout.d();
     // the history row and p come from sd
-    k_kj<<<1, kKjThreads, 0, m->st>>>(m->g, p - 1, A, m->cfg.relax, m->cfg.tol, m->cfg.maxiter,
-                                       m->frozen.d(), m->K.d(), gs, gt, kj, kj + NLL, kj + 2 * NLL,
-                                       m->perf.d(), (double)p, (int *)m->flag.p, sd);
+    k_kj_matrix<<<blocks((int64_t)NLL * NLL), 256, 0, m->st>>>(m->g, m->K.d(), m->Mkj.d());
+    WM_HIP(hipGetLastError());
+    k_kj<<<1, kKjThreads, m->kj_lds_bytes, m->st>>>(
+        m->g, p - 1, A, m->cfg.relax, m->cfg.tol, m->cfg.maxiter, m->frozen.d(), m->K.d(),
+        m->Mkj.d(), gs, gt, kj, kj + NLL, kj + 2 * NLL, m->perf.d(), (double)p,
+        (int *)m->flag.p, sd, m->kj_m_lds);
     WM_HIP(hipGetLastError());
     rc = push_hist(m, m->vel.d(), A);
     if (rc) return rc;
@@ -1519,6 +1624,11 @@ int winds_march_create(const winds_march_cfg *cfg, winds_march **out) {
     }
     // lifting-line sums have few targets: FP64 splits the segment range
     if (m->ll.precision == WINDS_PREC_FP64 && m->ll.split <= 1) m->ll.split = 256;
+    rc = kj_lds_plan(m, (c.nst - 1) * c.nb, c.n_alpha);
+    if (rc) {
+        delete m;
+        return rc;
+    }
     const int nst = c.nst, ns = c.nst - 1, nb = c.nb, na = c.n_alpha, nt = c.n_times;
     const int nf = c.n_foils;
     m->h_time.assign(c.time, c.time + nt);
@@ -1626,6 +1736,7 @@ int winds_march_create(const winds_march_cfg *cfg, winds_march **out) {
     WMC(m->llp.alloc((size_t)3 * NLL * sizeof(double)));
     WMC(m->frozen.alloc((size_t)3 * NLL * sizeof(double)));
     WMC(m->K.alloc((size_t)3 * NLL * NCH * sizeof(double)));
+    WMC(m->Mkj.alloc((size_t)3 * NLL * NLL * sizeof(double)));
     WMC(m->kjout.alloc((size_t)3 * NLL * sizeof(double)));
     WMC(m->perf.alloc((size_t)5 * nt * sizeof(double)));
     WMC(m->flag.alloc(256));
