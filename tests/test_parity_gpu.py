@@ -63,6 +63,27 @@ def test_fp64_matlab_cutoff_bitexact(c1):
     assert np.array_equal(u, ref)
 
 
+@pytest.mark.parametrize("m,p,q,nt", [(1, 1, 1, 1), (5, 1, 1, 3), (5, 3, 1, 64), (7, 9, 2, 65),
+                                      (3, 40, 3, 130), (18, 17, 1, 200)])
+def test_fp64_matlab_nested_sum_shapes_bitexact(need_gpu, m, p, q, nt):
+    """The nested sum sum(sum(sum(., 1), 3), 4) at shapes with fewer
+    columns than a workgroup's waves, a single column, partial rounds and
+    partial target groups: bitwise equal to the oracle's loop."""
+    rng = np.random.default_rng(1000 * m + 10 * p + q)
+    F1 = np.asfortranarray(rng.normal(size=(m, 3, p, q)))
+    F2 = np.asfortranarray(F1 + 0.3 * rng.normal(size=(m, 3, p, q)))
+    g = np.asfortranarray(rng.normal(size=(m, 1, p, q)))
+    rc = np.asfortranarray(0.01 + 0.01 * rng.random(size=(m, 1, p, q)))
+    P = np.asfortranarray(rng.normal(size=(nt, 3, 1, 1)) * 2.0)
+    for d in (1e-4, "visc2"):
+        u, _ = winds.BiotSavart(F1, F2, P, g, rc, d, 1000.0, "full")
+        ref = O.bs_matlab(F1, F2, P, g, rc, d, co=1000.0)
+        if d == 1e-4:
+            assert np.array_equal(u, ref), np.max(np.abs(u - ref))
+        else:
+            assert stats(u, ref)["max_rel"] <= 1e-12
+
+
 # --------------------------------------------------------------------------
 # FP32 compensated
 # --------------------------------------------------------------------------

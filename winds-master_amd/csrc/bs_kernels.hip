@@ -2,17 +2,19 @@
 //
 // Work decomposition: one target per lane, every lane of a wavefront walks
 // the same segment sequence, so each segment record is wave-uniform and is
-// fetched with scalar loads into SGPRs (no LDS staging, no barriers); the
-// VALU only does per-pair arithmetic.  Segment-only terms (L, |L|^2,
+// fetched with scalar loads into SGPRs (no LDS staging of records); the
+// VALU only does per-pair arithmetic.  The FP64 kernel adds LDS only to
+// hand terms between the waves that share 64 targets.  Segment-only terms (L, |L|^2,
 // 1/|L|^2, gamma/4pi, core constant, near radius) are hoisted into a
 // prepared record by prep_seg32.
 //
 // Kernels:
-//   bs_fp64_mex      FP64, biot_p2p arithmetic and the sequential segment
+//   bs_fp64_ordered  FP64, biot_p2p arithmetic with the sequential segment
 //                    order of BiotSavart_nogpu (BiotSavartMex.cu:63-93,
-//                    268-282); bitwise equal to the CPU restatement.
-//   bs_fp64_matlab   FP64, BiotSavart.m arithmetic (:86-115) and its nested
-//                    sum order sum(sum(sum(.,1),3),4).
+//                    268-282), or BiotSavart.m arithmetic (:86-115) with its
+//                    nested sum order sum(sum(sum(.,1),3),4); terms of 64
+//                    targets computed by many waves, added in that order by
+//                    one; bitwise equal to the CPU restatements.
 //   bs_fp32c         FP32 pair arithmetic with r = P-B taken in FP64 and
 //                    rounded once per filament point and carried to the next
 //                    segment of the filament as its r1, r1 x r2 evaluated as
@@ -29,6 +31,7 @@
 #include <float.h>
 #include <stdint.h>
 
+#include "bs_common.hpp"
 #include "bs_fp32c.hpp"
 #include "bs_kernels.hpp"
 #include "bs_pair.hpp"
@@ -39,54 +42,112 @@ namespace winds {
 // FP64 kernels
 // ---------------------------------------------------------------------------
 
-// Scheduled for 1-2 waves per SIMD (the sequential sum allows one lane per
-// target, ~2 waves per SIMD at 133,920 targets): the compiler then spends
-// registers on overlapping the four terms' square roots and divisions
-// (168 VGPRs; 12% faster than at its default 4-wave register budget).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2))) void bs_fp64_mex(
-    const double *__restrict__ tx, const double *__restrict__ ty,
-    const double *__restrict__ tz, int64_t nt, const Seg64 *__restrict__ seg,
-    int64_t nseg, double d, double *__restrict__ ux, double *__restrict__ uy,
-    double *__restrict__ uz) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    fp64_range<0, 0>(seg, 0, nseg, tx[t], ty[t], tz[t], d, 0, 0.0, sx, sy, sz);
-    ux[t] = sx;
-    uy[t] = sy;
-    uz[t] = sz;
+// FP64 direct sum in the reference's order, with the terms of one target
+// computed by many waves.  A workgroup takes 64 targets (one per lane); in
+// each round its W waves compute kOrdTerms consecutive segments' terms each
+// into LDS, and one wave (wave r mod W in round r, so the work falls on
+// every SIMD) adds the round's W * kOrdTerms terms in segment order to the
+// running sums, which it hands on through LDS.  The sums follow
+// BiotSavart.m's nesting sum(sum(sum(., 1), 3), 4): columns of col segments
+// summed from zero, column sums added into the sum of their block of blk
+// segments, block sums into the total; col = blk = nseg is the plain
+// sequential sum of BiotSavart_nogpu (BiotSavartMex.cu:268-282), since a
+// sum that starts at +0 never is -0 and 0 + x = x.  Every addition is the
+// reference's, so results are bitwise those of one lane walking all
+// segments, while W waves per 64 targets keep the FP64 pipeline fed where
+// one lane per target would leave ~2 waves per SIMD.  MODE 0 biot_p2p (MEX),
+// 1 BiotSavart.m cut-off, 2 Vatistas of order NF (1, 2) or n (NF = 0).
+// Dynamic LDS: ord_lds_bytes(W).
+constexpr int kOrdTerms = 2;
+__host__ __device__ constexpr size_t ord_lds_doubles(int W) {
+    return (size_t)2 * W * kOrdTerms * 3 * 64 + 9 * 64;
 }
-
-// col = segments per station column, blk = segments per blade (col * ages).
-// MODE 1 cut-off, 2 Vatistas of order NF (1, 2) or n (NF = 0).
 template <int MODE, int NF>
-__global__ __launch_bounds__(1024) void bs_fp64_matlab(
+__global__ __launch_bounds__(1024) void bs_fp64_ordered(
     const double *__restrict__ tx, const double *__restrict__ ty,
     const double *__restrict__ tz, int64_t nt, const Seg64 *__restrict__ seg,
     int64_t nseg, int64_t col, int64_t blk, double d, int n, double co,
     double *__restrict__ ux, double *__restrict__ uy, double *__restrict__ uz) {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nt) return;
-    const double px = tx[t], py = ty[t], pz = tz[t];
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int64_t b0 = 0; b0 < nseg; b0 += blk) {
-        const int64_t b1 = b0 + blk < nseg ? b0 + blk : nseg;
-        double bx = 0.0, by = 0.0, bz = 0.0;
-        for (int64_t c0 = b0; c0 < b1; c0 += col) {
-            const int64_t c1 = c0 + col < b1 ? c0 + col : b1;
-            double cx = 0.0, cy = 0.0, cz = 0.0;
-            fp64_range<MODE, NF>(seg, c0, c1, px, py, pz, d, n, co, cx, cy, cz);
-            bx += cx;
-            by += cy;
-            bz += cz;
+    extern __shared__ double s_ord[];
+    const int W = blockDim.x >> 6;
+    const int per = W * kOrdTerms;  // terms per round
+    double *s_acc = s_ord + (size_t)2 * per * 3 * 64;  // [9][64] running sums
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t t = (int64_t)blockIdx.x * 64 + lane;
+    const bool live = t < nt;
+    const double px = live ? tx[t] : 0.0, py = live ? ty[t] : 0.0, pz = live ? tz[t] : 0.0;
+    if (wave == 0)
+        for (int i = 0; i < 9; ++i) s_acc[i * 64 + lane] = 0.0;
+    const int64_t rounds = (nseg + per - 1) / per;
+    for (int64_t r = 0; r < rounds; ++r) {
+        double *buf = s_ord + (size_t)(r & 1) * per * 3 * 64;  // double-buffered by round
+        {
+            const int64_t k0 = r * per + (int64_t)wave * kOrdTerms;
+            double tt[kOrdTerms][3];
+#pragma unroll
+            for (int i = 0; i < kOrdTerms; ++i) {
+                tt[i][0] = tt[i][1] = tt[i][2] = 0.0;
+                if (k0 + i < nseg)
+                    fp64_term<MODE, NF>(seg[k0 + i], px, py, pz, d, n, co, tt[i][0], tt[i][1],
+                                        tt[i][2]);
+            }
+#pragma unroll
+            for (int i = 0; i < kOrdTerms; ++i) {
+                double *o = buf + (size_t)(wave * kOrdTerms + i) * 3 * 64 + lane;
+                o[0] = tt[i][0];
+                o[64] = tt[i][1];
+                o[128] = tt[i][2];
+            }
         }
-        sx += bx;
-        sy += by;
-        sz += bz;
+        __syncthreads();
+        if (wave == (int)(r % W)) {
+            double cx = s_acc[lane], cy = s_acc[64 + lane], cz = s_acc[128 + lane];
+            double bx = s_acc[192 + lane], by = s_acc[256 + lane], bz = s_acc[320 + lane];
+            double sx = s_acc[384 + lane], sy = s_acc[448 + lane], sz = s_acc[512 + lane];
+            const int64_t k0 = r * per;
+            int64_t kb = k0 % blk, kc = kb % col;  // offsets in the block and the column
+            const int m = nseg - k0 < per ? (int)(nseg - k0) : per;
+            for (int j = 0; j < m; ++j) {
+                if (kc == 0 && k0 + j > 0) {  // a column begins: close the previous one
+                    bx += cx;
+                    by += cy;
+                    bz += cz;
+                    cx = cy = cz = 0.0;
+                    if (kb == 0) {  // and a block
+                        sx += bx;
+                        sy += by;
+                        sz += bz;
+                        bx = by = bz = 0.0;
+                    }
+                }
+                const double *q = buf + (size_t)j * 3 * 64 + lane;
+                cx += q[0];
+                cy += q[64];
+                cz += q[128];
+                ++kb;
+                ++kc;
+                if (kb == blk)
+                    kb = kc = 0;
+                else if (kc == col)
+                    kc = 0;
+            }
+            s_acc[lane] = cx;
+            s_acc[64 + lane] = cy;
+            s_acc[128 + lane] = cz;
+            s_acc[192 + lane] = bx;
+            s_acc[256 + lane] = by;
+            s_acc[320 + lane] = bz;
+            s_acc[384 + lane] = sx;
+            s_acc[448 + lane] = sy;
+            s_acc[512 + lane] = sz;
+        }
     }
-    ux[t] = sx;
-    uy[t] = sy;
-    uz[t] = sz;
+    __syncthreads();
+    if (wave == 0 && live) {
+        ux[t] = (s_acc[384 + lane] + (s_acc[192 + lane] + s_acc[lane]));
+        uy[t] = (s_acc[448 + lane] + (s_acc[256 + lane] + s_acc[64 + lane]));
+        uz[t] = (s_acc[512 + lane] + (s_acc[320 + lane] + s_acc[128 + lane]));
+    }
 }
 
 // FP64 over one segment chunk per grid.y (split-K for few targets); the
@@ -395,6 +456,27 @@ hipError_t launch_zero(double *p, int64_t n, hipStream_t st) {
     return hipGetLastError();
 }
 
+// Lets bs_fp64_ordered<MODE, NF> take its largest dynamic LDS (once per
+// process; thread-safe static initialisation).
+template <int MODE, int NF>
+static hipError_t ord_allow_lds() {
+    static const hipError_t rc =
+        hipFuncSetAttribute((const void *)bs_fp64_ordered<MODE, NF>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)(ord_lds_doubles(16) * sizeof(double)));
+    return rc;
+}
+
+// Waves per workgroup of bs_fp64_ordered.  16 while the target groups are
+// fewer than two per CU (more waves per group fill the chip); 8 above, where
+// two workgroups share a CU's LDS and keep four waves per SIMD (measured on
+// the 3 x 31-station wakes: 1,440 ages 245 ms at 8 against 341 ms at 16,
+// 72 ages 1.85 ms at 16 against 2.28 ms at 8, Vatistas-2).
+static int ordered_waves(int64_t groups) {
+    const int64_t cus = device_cu_count() > 0 ? device_cu_count() : 256;
+    return groups < 2 * cus ? 16 : 8;
+}
+
 hipError_t launch_fp64(const double *t3, int64_t nt, const Seg64 *seg,
                        int64_t nseg, const EvalCfg &c, double *u3, hipStream_t st,
                        double *part) {
@@ -429,25 +511,32 @@ hipError_t launch_fp64(const double *t3, int64_t nt, const Seg64 *seg,
         bs_reduce_split<<<grid_for(nt, 256), 256, 0, st>>>(part, nt, nsplit, ux, uy, uz);
         return hipGetLastError();
     }
-    if (c.semantics == 0) {
-        bs_fp64_mex<<<grid_for(nt, block256), block256, 0, st>>>(tx, ty, tz, nt, seg, nseg, c.d, ux,
-                                                                 uy, uz);
-    } else {
-        const int64_t col = c.seg_col > 0 ? c.seg_col : nseg;
-        const int64_t blk = c.seg_blk > 0 ? c.seg_blk : nseg;
+    {
+        const bool mex = c.semantics == 0;
+        const int64_t col = !mex && c.seg_col > 0 ? c.seg_col : nseg;
+        const int64_t blk = !mex && c.seg_blk > 0 ? c.seg_blk : nseg;
         const int n = c.core == 0 ? 0 : c.vat_n;
-#define WINDS_MATLAB(M, F)                                                                   \
-    bs_fp64_matlab<M, F><<<grid_for(nt, block), block, 0, st>>>(tx, ty, tz, nt, seg, nseg, col, \
-                                                                blk, c.d, n, c.co, ux, uy, uz)
-        if (n == 0)
-            WINDS_MATLAB(1, 0);
+        const int64_t groups = (nt + 63) / 64;
+        const int W = ordered_waves(groups);
+        const size_t lds = ord_lds_doubles(W) * sizeof(double);
+#define WINDS_ORD(M, F)                                                                         \
+    do {                                                                                        \
+        hipError_t ea = ord_allow_lds<M, F>();                                                  \
+        if (ea != hipSuccess) return ea;                                                        \
+        bs_fp64_ordered<M, F><<<(unsigned)groups, 64 * W, lds, st>>>(                           \
+            tx, ty, tz, nt, seg, nseg, col, blk, c.d, n, c.co, ux, uy, uz);                     \
+    } while (0)
+        if (mex)
+            WINDS_ORD(0, 0);
+        else if (n == 0)
+            WINDS_ORD(1, 0);
         else if (n == 1)
-            WINDS_MATLAB(2, 1);
+            WINDS_ORD(2, 1);
         else if (n == 2)
-            WINDS_MATLAB(2, 2);
+            WINDS_ORD(2, 2);
         else
-            WINDS_MATLAB(2, 0);
-#undef WINDS_MATLAB
+            WINDS_ORD(2, 0);
+#undef WINDS_ORD
     }
     return hipGetLastError();
 }
