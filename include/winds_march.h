@@ -155,8 +155,11 @@ const double *winds_march_uind_dev(const winds_march *m);
  *       direct far sum (WINDS_SURR_DIRECT) or nothing (WINDS_SURR_DROP);
  *   older nodes: the induced velocity of the node of the same age at the
  *       last check, held in the frame turning with the rotor.
- * Every check_every steps (and on the first such step) the step is a full
- * direct sum; it refreshes the held velocities and, in WINDS_SURR_MLP,
+ * Every check_every steps (and on the first such step) the step's wake
+ * evaluation at time p is a full direct sum (with PCC the corrector's
+ * evaluation at the predicted nodes then goes through the surrogate as on
+ * other steps, except with check_every = 1, which is the plain march); it
+ * refreshes the held velocities and, in WINDS_SURR_MLP,
  * compares the network with the direct far sum at the near targets and the
  * lifting line: error = max |u_net - u_far| / max |u_ind| over each point
  * set, u_ind being the whole wake's induced velocity there (the velocity the

@@ -1439,10 +1439,16 @@ int step_body(winds_march *m, StepPlan &sp) {
                 rc = vcore(m, p + 1, Xn, A_new, m->gs1.d(), m->gt1.d(), lnow, lpred, m->rs1.d(),
                            m->rt1.d());
             const int64_t Nn = nodes_of(m, A_new);
+            // a check step's corrector goes through the surrogate too (the
+            // check has just refreshed the held far-wake velocities and
+            // decided between the network and the direct far sum), unless
+            // every step is a check: that march stays the plain one
+            const How how_c = how == kDirect || (how == kDirectCheck && m->sc.check_every <= 1)
+                                  ? kDirect
+                                  : kSurrogate;
             if (!rc)
                 rc = wake_velocity(m, Xn, A_new, m->gs1.d(), m->gt1.d(), m->rs1.d(), m->rt1.d(),
-                                   t + 1, m->uind.d(),
-                                   how == kSurrogate ? kSurrogate : kDirect);
+                                   t + 1, m->uind.d(), how_c);
             if (!rc) rc = add_wind(m, p + 1, Nn, m->uind.d(), m->velp.d());
             if (!rc) rc = advect(m, X, A, m->vel.d(), m->velp.d(), 0, p + 1, Xn, A_new);
         }
