@@ -232,7 +232,28 @@ __global__ void bs_reduce_guard(const double *__restrict__ part, const float *__
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     double sx = 0.0, sy = 0.0, sz = 0.0, a2 = 0.0;
-    for (int s = 0; s < split; ++s) {
+    int s = 0;
+    // eight partials' loads issued together, then added in order
+    for (; s + 8 <= split; s += 8) {
+        double v[8][3];
+        float e[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const double *p = part + (int64_t)(s + i) * 3 * nt;
+            v[i][0] = p[t];
+            v[i][1] = p[nt + t];
+            v[i][2] = p[2 * nt + t];
+            e[i] = est[(int64_t)(s + i) * nt + t];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            sx += v[i][0];
+            sy += v[i][1];
+            sz += v[i][2];
+            a2 += (double)e[i];
+        }
+    }
+    for (; s < split; ++s) {
         const double *p = part + (int64_t)s * 3 * nt;
         sx += p[t];
         sy += p[nt + t];
@@ -365,7 +386,25 @@ __global__ void bs_reduce_split(const double *__restrict__ part, int64_t nt,
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nt) return;
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int s = 0; s < split; ++s) {
+    int s = 0;
+    // eight partials' loads issued together, then added in order
+    for (; s + 8 <= split; s += 8) {
+        double v[8][3];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const double *p = part + (int64_t)(s + i) * 3 * nt;
+            v[i][0] = p[t];
+            v[i][1] = p[nt + t];
+            v[i][2] = p[2 * nt + t];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            sx += v[i][0];
+            sy += v[i][1];
+            sz += v[i][2];
+        }
+    }
+    for (; s < split; ++s) {
         const double *p = part + (int64_t)s * 3 * nt;
         sx += p[t];
         sy += p[nt + t];
@@ -374,6 +413,47 @@ __global__ void bs_reduce_split(const double *__restrict__ part, int64_t nt,
     ux[t] = sx;
     uy[t] = sy;
     uz[t] = sz;
+}
+
+// The same sums (and, with est, bs_reduce_guard's test) for few targets and
+// many partials: one wave per target stages the target's partials in LDS
+// with all its lanes, then lanes 0-3 add the components and the estimate in
+// split order -- the same additions as one thread per target, without a
+// chain of dependent device-memory loads.  Dynamic LDS: 4 * split doubles.
+__global__ __launch_bounds__(64) void bs_reduce_wave(const double *__restrict__ part,
+                                                     const float *__restrict__ est, int64_t nt,
+                                                     int split, double kappa,
+                                                     double *__restrict__ ux,
+                                                     double *__restrict__ uy,
+                                                     double *__restrict__ uz,
+                                                     uint32_t *__restrict__ list,
+                                                     int *__restrict__ count) {
+    extern __shared__ double s_red[];  // [4][split]
+    const int64_t t = blockIdx.x;
+    const int lane = threadIdx.x;
+    for (int s = lane; s < split; s += 64) {
+        const double *p = part + (int64_t)s * 3 * nt + t;
+        s_red[s] = p[0];
+        s_red[split + s] = p[nt];
+        s_red[2 * split + s] = p[2 * nt];
+        s_red[3 * split + s] = est ? (double)est[(int64_t)s * nt + t] : 0.0;
+    }
+    __syncthreads();
+    double acc = 0.0;
+    if (lane < 4) {
+        const double *q = s_red + (int64_t)lane * split;
+        for (int s = 0; s < split; ++s) acc += q[s];
+    }
+    // lanes 0-2 hold the components, lane 3 the sum of squares
+    const double sx = __shfl(acc, 0), sy = __shfl(acc, 1), sz = __shfl(acc, 2),
+                 a2 = __shfl(acc, 3);
+    if (lane == 0) {
+        ux[t] = sx;
+        uy[t] = sy;
+        uz[t] = sz;
+        if (est && !(sqrt(a2) <= kappa * sqrt(sx * sx + sy * sy + sz * sz)))
+            list[atomicAdd(count, 1)] = (uint32_t)t;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -450,6 +530,26 @@ static inline unsigned grid_for(int64_t n, int block) {
     return (unsigned)((n + block - 1) / block);
 }
 
+// Split reduction by the kernel that suits the shape: one wave per target
+// while the targets are few (16 waves' worth of threads, one per target,
+// would leave the chip idle) and the partials fit in LDS, one thread per
+// target otherwise; the sums are the same either way.
+static hipError_t reduce_split(const double *part, const float *est, int64_t nt, int split,
+                               double kappa, double *u3, uint32_t *list, int *count,
+                               hipStream_t st) {
+    if (nt < 1024 && split <= 2048) {
+        bs_reduce_wave<<<(unsigned)nt, 64, (size_t)4 * split * sizeof(double), st>>>(
+            part, est, nt, split, kappa, u3, u3 + nt, u3 + 2 * nt, list, count);
+    } else if (est) {
+        bs_reduce_guard<<<grid_for(nt, 256), 256, 0, st>>>(part, est, nt, split, kappa, u3,
+                                                           u3 + nt, u3 + 2 * nt, list, count);
+    } else {
+        bs_reduce_split<<<grid_for(nt, 256), 256, 0, st>>>(part, nt, split, u3, u3 + nt,
+                                                           u3 + 2 * nt);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_zero(double *p, int64_t n, hipStream_t st) {
     if (n <= 0) return hipSuccess;
     fill_zero<<<grid_for(n, 256), 256, 0, st>>>(p, n);
@@ -508,8 +608,7 @@ hipError_t launch_fp64(const double *t3, int64_t nt, const Seg64 *seg,
 #undef WINDS_SPLIT
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        bs_reduce_split<<<grid_for(nt, 256), 256, 0, st>>>(part, nt, nsplit, ux, uy, uz);
-        return hipGetLastError();
+        return reduce_split(part, nullptr, nt, nsplit, 0.0, u3, nullptr, nullptr, st);
     }
     {
         const bool mex = c.semantics == 0;
@@ -695,16 +794,11 @@ hipError_t launch_fp32c(const double *t3, int64_t nt, const Seg64 *seg,
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (guard) {
-        bs_reduce_guard<<<grid_for(nt, 256), 256, 0, st>>>(part, g.est, nt, nsplit, c.guard, u3,
-                                                           u3 + nt, u3 + 2 * nt, g.list, g.count);
-        e = hipGetLastError();
+        e = reduce_split(part, g.est, nt, nsplit, c.guard, u3, g.list, g.count, st);
         if (e != hipSuccess) return e;
         return launch_guard_fix(t3, nt, seg, nullptr, 0, WakeSpan{0, nt, 0, nseg}, c, g, u3, st);
     }
-    if (nsplit > 1) {
-        bs_reduce_split<<<grid_for(nt, 256), 256, 0, st>>>(part, nt, nsplit, u3, u3 + nt, u3 + 2 * nt);
-        e = hipGetLastError();
-    }
+    if (nsplit > 1) e = reduce_split(part, nullptr, nt, nsplit, 0.0, u3, nullptr, nullptr, st);
     return e;
 }
 

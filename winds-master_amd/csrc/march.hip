@@ -197,29 +197,57 @@ __global__ void k_kelvin_trail(int nst, int nb, const double *__restrict__ gs_ol
 }
 
 // Kelvin, shed part: age 0 = bound; age a>=1 = Cz(a) - C(a-1), C = span
-// cumulative sum of trailing ages, Cz(A-1) = 0 (WInDS.m:139-140)
-__global__ void k_kelvin_shed(int nst, int nb, const double *__restrict__ gs_old, int A_old,
-                              const double *__restrict__ gt_new, int A_new,
-                              double *__restrict__ gs_new) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// cumulative sum of trailing ages, Cz(A-1) = 0 (WInDS.m:139-140).  A block
+// takes kShedAges ages of one blade: the trailing rows it reads and the
+// shed rows it writes are contiguous, so they pass through LDS with
+// coalesced accesses, and each thread runs its age's cumulative sums in
+// station order.  Dynamic LDS: kelvin_shed_lds(nst) bytes.
+constexpr int kShedAges = 32;
+inline size_t kelvin_shed_lds(int nst) {
+    return ((size_t)(kShedAges + 1) * nst + (size_t)kShedAges * (nst - 1)) * sizeof(double);
+}
+__global__ __launch_bounds__(kShedAges) void k_kelvin_shed(int nst, int nb,
+                                                           const double *__restrict__ gs_old,
+                                                           int A_old,
+                                                           const double *__restrict__ gt_new,
+                                                           int A_new,
+                                                           double *__restrict__ gs_new) {
+    extern __shared__ double s_kel[];
     const int ns = nst - 1;
-    if (i >= A_new * nb) return;
-    const int a = i % A_new, b = i / A_new;
-    double *out = gs_new + (int64_t)b * ns * A_new + (int64_t)a * ns;
-    if (a == 0) {
-        const double *gb = gs_old + (int64_t)b * ns * A_old;
-        for (int m = 0; m < ns; ++m) out[m] = gb[m];
-        return;
-    }
     const int nta = A_new - 1;  // trailing ages
-    const double *tp = gt_new + (int64_t)b * nst * nta + (int64_t)(a - 1) * nst;
-    const double *tc = a <= nta - 1 ? tp + nst : nullptr;
-    double cprev = 0.0, ccur = 0.0;
-    for (int m = 0; m < ns; ++m) {
-        cprev += tp[m];
-        if (tc) ccur += tc[m];
-        out[m] = (tc ? ccur : 0.0) - cprev;
+    const int blocks_per_blade = (A_new + kShedAges - 1) / kShedAges;
+    const int b = blockIdx.x / blocks_per_blade;
+    const int a0 = (blockIdx.x % blocks_per_blade) * kShedAges;
+    const int na = A_new - a0 < kShedAges ? A_new - a0 : kShedAges;  // ages of this block
+    double *tr = s_kel;                               // trailing rows a0-1 .. a0+na-1
+    double *sh = s_kel + (size_t)(kShedAges + 1) * nst;  // shed rows a0 .. a0+na-1
+    const double *gtb = gt_new + (int64_t)b * nst * nta;
+    const int r0 = a0 - 1 > 0 ? a0 - 1 : 0;           // first trailing row read
+    const int r1 = a0 + na < nta ? a0 + na : nta;     // one past the last
+    for (int k = threadIdx.x; k < (r1 - r0) * nst; k += kShedAges)
+        tr[(r0 - (a0 - 1)) * nst + k] = gtb[(int64_t)r0 * nst + k];
+    __syncthreads();
+    const int i = threadIdx.x;
+    if (i < na) {
+        const int a = a0 + i;
+        double *out = sh + (size_t)i * ns;
+        if (a == 0) {
+            const double *gb = gs_old + (int64_t)b * ns * A_old;
+            for (int m = 0; m < ns; ++m) out[m] = gb[m];
+        } else {
+            const double *tp = tr + (size_t)i * nst;  // trailing age a-1
+            const double *tc = a <= nta - 1 ? tp + nst : nullptr;
+            double cprev = 0.0, ccur = 0.0;
+            for (int m = 0; m < ns; ++m) {
+                cprev += tp[m];
+                if (tc) ccur += tc[m];
+                out[m] = (tc ? ccur : 0.0) - cprev;
+            }
+        }
     }
+    __syncthreads();
+    double *dst = gs_new + (int64_t)b * ns * A_new + (int64_t)a0 * ns;
+    for (int k = threadIdx.x; k < na * ns; k += kShedAges) dst[k] = sh[k];
 }
 
 // vcore.m:30-51 + FilamentLength.m + filamentmod.m:27-41 for one time index
@@ -1054,9 +1082,9 @@ int kelvin(winds_march *m, const double *gs_old, int A_old, const double *gt_old
     k_kelvin_trail<<<blocks(nt), 256, 0, m->st>>>(m->g.nst, m->g.nb, gs_old, A_old, gt_old, A_new,
                                                   gt_new);
     WM_HIP(hipGetLastError());
-    k_kelvin_shed<<<blocks((int64_t)A_new * m->g.nb), 256, 0, m->st>>>(m->g.nst, m->g.nb, gs_old,
-                                                                         A_old, gt_new, A_new,
-                                                                         gs_new);
+    const int shed_blocks = (A_new + kShedAges - 1) / kShedAges * m->g.nb;
+    k_kelvin_shed<<<shed_blocks, kShedAges, kelvin_shed_lds(m->g.nst), m->st>>>(
+        m->g.nst, m->g.nb, gs_old, A_old, gt_new, A_new, gs_new);
     WM_HIP(hipGetLastError());
     return WINDS_OK;
 }
@@ -1395,6 +1423,15 @@ int kj_lds_plan(winds_march *m, int NLL, int n_alpha) {
     return 0;
 }
 
+// The Kelvin shed kernel's LDS for nst stations (up to 130 KB at the
+// largest blade).
+int kelvin_lds_plan(int nst) {
+    WM_HIP(hipFuncSetAttribute((const void *)k_kelvin_shed,
+                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)kelvin_shed_lds(nst)));
+    return 0;
+}
+
 int step_body(winds_march *m, StepPlan &sp) {
     const int p = m->p;
     const int t = p;
@@ -1629,8 +1666,9 @@ int winds_march_create(const winds_march_cfg *cfg, winds_march **out) {
         return rc;
     }
     // lifting-line sums have few targets: FP64 splits the segment range
-    if (m->ll.precision == WINDS_PREC_FP64 && m->ll.split <= 1) m->ll.split = 256;
+    if (m->ll.precision == WINDS_PREC_FP64 && m->ll.split <= 1) m->ll.split = 512;
     rc = kj_lds_plan(m, (c.nst - 1) * c.nb, c.n_alpha);
+    if (!rc) rc = kelvin_lds_plan(c.nst);
     if (rc) {
         delete m;
         return rc;
