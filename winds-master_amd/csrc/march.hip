@@ -136,9 +136,15 @@ __device__ double interp1(const double *xp, const double *fp, int n, double x) {
     return slope * (x - xp[lo]) + fp[lo];
 }
 
-// interp1 of two tables over the same abscissae, with one search
-__device__ __forceinline__ void interp1x2(const double *xp, const double *fa, const double *fb,
-                                          int n, double x, double &ya, double &yb) {
+// interp1 of two tables over the same abscissae (MATLAB interp1 'linear':
+// NaN outside [xp[0], xp[n-1]]), with one search for both.  The search
+// starts from the interval of the previous call (hint; -1 for none): the
+// interval with xp[lo] <= x < xp[lo + 1] is unique, so a hint that brackets
+// x gives the binary search's answer; otherwise the search runs.  sa, sb
+// hold the hinted interval's slopes (set whenever the hint changes).
+__device__ __forceinline__ void interp1x2_hint(const double *xp, const double *fa,
+                                               const double *fb, int n, double x, double &ya,
+                                               double &yb, int &hint, double &sa, double &sb) {
     if (!(x >= xp[0] && x <= xp[n - 1])) {
         ya = yb = __builtin_nan("");
         return;
@@ -148,17 +154,25 @@ __device__ __forceinline__ void interp1x2(const double *xp, const double *fa, co
         yb = fb[n - 1];
         return;
     }
-    int lo = 0, hi = n - 1;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (xp[mid] <= x)
-            lo = mid;
-        else
-            hi = mid;
+    int lo = hint;
+    if (!(lo >= 0 && lo < n - 1 && xp[lo] <= x && x < xp[lo + 1])) {
+        lo = 0;
+        int hi = n - 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (xp[mid] <= x)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        hint = lo;
+        // the interval's slopes, kept while the interval stays
+        const double dx = xp[lo + 1] - xp[lo];
+        sa = (fa[lo + 1] - fa[lo]) / dx;
+        sb = (fb[lo + 1] - fb[lo]) / dx;
     }
-    const double dx = xp[lo + 1] - xp[lo];
-    ya = (fa[lo + 1] - fa[lo]) / dx * (x - xp[lo]) + fa[lo];
-    yb = (fb[lo + 1] - fb[lo]) / dx * (x - xp[lo]) + fb[lo];
+    ya = sa * (x - xp[lo]) + fa[lo];
+    yb = sb * (x - xp[lo]) + fb[lo];
 }
 
 // initials.m:110-125: bound circulation 0.5*|U(1)|*chord*cl, cl the
@@ -658,6 +672,7 @@ __host__ __device__ inline KjLds kj_lds(int NLL, int n_alpha, bool with_m) {
 // reads the polar of its airfoil (blade.NFoil, KuttaJoukowski.m:262-269).
 // Writes the bound circulation, the trailing age 0 and shed age 1
 // circulation, cl/cd/aoa and one history row.
+template <bool MLDS>
 __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double relax,
                                                   double tol, int maxiter,
                                                   const double *__restrict__ frozen,
@@ -669,7 +684,7 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double
                                                   double *__restrict__ aoa_out,
                                                   double *__restrict__ hist_row, double p_index,
                                                   int *__restrict__ flag,
-                                                  const StepDev *__restrict__ sd, int m_lds) {
+                                                  const StepDev *__restrict__ sd) {
     // sd: the table index, the history row (hist_row is then the history's
     // start) and p from the step's device values
     if (sd) {
@@ -682,14 +697,14 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double
     const int per = 2 * ns + nst;
     const int NCH = per * nb;
     extern __shared__ double s_dyn[];
-    const KjLds L = kj_lds(NLL, g.n_alpha, m_lds != 0);
+    const KjLds L = kj_lds(NLL, g.n_alpha, MLDS);
     double *s_g = s_dyn + L.g;        // bound circulation
     double *s_c1 = s_dyn + L.c1;      // span cumsum of trailing age 1
     double *s_part = s_dyn + L.part;  // partial sums of the changing filaments
     double *s_al = s_dyn + L.al, *s_tcl = s_dyn + L.tcl, *s_tcd = s_dyn + L.tcd;  // the polars
     double *s_fz = s_dyn + L.fz;  // the frozen filaments' velocity at the points
     double *s_ll = s_dyn + L.ll;  // per point: chord frame, inflow, |inflow|, chord
-    double *s_mp = s_dyn + L.mp;  // M in the chord frames (m_lds)
+    double *s_mp = s_dyn + L.mp;  // M in the chord frames (MLDS)
     __shared__ double s_red[kKjWaves];
     __shared__ double s_red2[kKjWaves];
     __shared__ int s_foff[kMaxFoils], s_flen[kMaxFoils];
@@ -776,7 +791,7 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double
     }
     // The changing filaments' share from the bound values: they split into
     // kparts ranges, one thread per (range, point ki); each term is M's
-    // column for (ki, j) turned into ki's chord frame.  With m_lds the terms
+    // column for (ki, j) turned into ki's chord frame.  With MLDS the terms
     // are formed once into LDS; otherwise they are formed from M at every
     // iteration, with the same arithmetic.
     const int jlo = (int)((int64_t)kpart * NLL / kparts);
@@ -791,7 +806,7 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double
         byk[0] = q[3];
         byk[1] = q[4];
         byk[2] = q[5];
-        if (m_lds)
+        if constexpr (MLDS)
             for (int jj = jlo; jj < jhi; ++jj) {
                 const double *Mj = M + (int64_t)jj * 3 * NLL + ki;
                 const double a = Mj[0], b = Mj[NLL], c = Mj[2 * NLL];
@@ -802,11 +817,29 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double
     __syncthreads();
     double dgmax = 1.0;
     int j = 0;
+    int hint = -1;  // the polar interval of point tid at the last iteration
+    double sl_a = 0.0, sl_b = 0.0;  // and its cl, cd slopes
     while (dgmax > tol && j < maxiter) {
         if (active) {
             double px = 0.0, py = 0.0;
-            if (m_lds) {
-                for (int jj = jlo; jj < jhi; ++jj) {
+            if constexpr (MLDS) {
+                // four bound values' LDS reads issued together, added in order
+                int jj = jlo;
+                for (; jj + 4 <= jhi; jj += 4) {
+                    double mx[4], my[4], gj[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        gj[q] = s_g[jj + q];
+                        mx[q] = s_mp[2 * (jj + q) * NLL + ki];
+                        my[q] = s_mp[(2 * (jj + q) + 1) * NLL + ki];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        px += mx[q] * gj[q];
+                        py += my[q] * gj[q];
+                    }
+                }
+                for (; jj < jhi; ++jj) {
                     const double gj = s_g[jj];
                     px += s_mp[2 * jj * NLL + ki] * gj;
                     py += s_mp[(2 * jj + 1) * NLL + ki] * gj;
@@ -840,7 +873,8 @@ __global__ __launch_bounds__(kKjThreads) void k_kj(Geom g, int p0, int A, double
             double cl, cd;
             const int f = g.foil[i % ns];
             const int o = s_foff[f];
-            interp1x2(s_al + o, s_tcl + o, s_tcd + o, s_flen[f], aoa, cl, cd);
+            interp1x2_hint(s_al + o, s_tcl + o, s_tcd + o, s_flen[f], aoa, cl, cd, hint, sl_a,
+                           sl_b);
             if (cl != cl) s_nan = 1;
             const double gn = 0.5 * Vinf * chord * cl;
             const double dg = gn - s_g[i];
@@ -1409,8 +1443,10 @@ int kj_lds_plan(winds_march *m, int NLL, int n_alpha) {
     int dev = 0, lim = 0;
     WM_HIP(hipGetDevice(&dev));
     WM_HIP(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    hipFuncAttributes fa;
-    WM_HIP(hipFuncGetAttributes(&fa, (const void *)k_kj));
+    hipFuncAttributes fa, fb;
+    WM_HIP(hipFuncGetAttributes(&fa, (const void *)k_kj<true>));
+    WM_HIP(hipFuncGetAttributes(&fb, (const void *)k_kj<false>));
+    fa.sharedSizeBytes = std::max(fa.sharedSizeBytes, fb.sharedSizeBytes);
     const size_t avail = (size_t)lim > fa.sharedSizeBytes ? (size_t)lim - fa.sharedSizeBytes : 0;
     const size_t with_m = (size_t)kj_lds(NLL, n_alpha, true).total * sizeof(double);
     const size_t without = (size_t)kj_lds(NLL, n_alpha, false).total * sizeof(double);
@@ -1418,8 +1454,8 @@ int kj_lds_plan(winds_march *m, int NLL, int n_alpha) {
     m->kj_lds_bytes = m->kj_m_lds ? with_m : without;
     if (m->kj_lds_bytes > avail)
         return set_error(WINDS_E_ARG, "march: the lifting line needs more LDS than a workgroup has");
-    WM_HIP(hipFuncSetAttribute((const void *)k_kj, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)m->kj_lds_bytes));
+    WM_HIP(hipFuncSetAttribute(m->kj_m_lds ? (const void *)k_kj<true> : (const void *)k_kj<false>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)m->kj_lds_bytes));
     return 0;
 }
 
@@ -1516,10 +1552,16 @@ int step_body(winds_march *m, StepPlan &sp) {
     // the history row and p come from sd
     k_kj_matrix<<<blocks((int64_t)NLL * NLL), 256, 0, m->st>>>(m->g, m->K.d(), m->Mkj.d());
     WM_HIP(hipGetLastError());
-    k_kj<<<1, kKjThreads, m->kj_lds_bytes, m->st>>>(
-        m->g, p - 1, A, m->cfg.relax, m->cfg.tol, m->cfg.maxiter, m->frozen.d(), m->K.d(),
-        m->Mkj.d(), gs, gt, kj, kj + NLL, kj + 2 * NLL, m->perf.d(), (double)p,
-        (int *)m->flag.p, sd, m->kj_m_lds);
+    if (m->kj_m_lds)
+        k_kj<true><<<1, kKjThreads, m->kj_lds_bytes, m->st>>>(
+            m->g, p - 1, A, m->cfg.relax, m->cfg.tol, m->cfg.maxiter, m->frozen.d(), m->K.d(),
+            m->Mkj.d(), gs, gt, kj, kj + NLL, kj + 2 * NLL, m->perf.d(), (double)p,
+            (int *)m->flag.p, sd);
+    else
+        k_kj<false><<<1, kKjThreads, m->kj_lds_bytes, m->st>>>(
+            m->g, p - 1, A, m->cfg.relax, m->cfg.tol, m->cfg.maxiter, m->frozen.d(), m->K.d(),
+            m->Mkj.d(), gs, gt, kj, kj + NLL, kj + 2 * NLL, m->perf.d(), (double)p,
+            (int *)m->flag.p, sd);
     WM_HIP(hipGetLastError());
     rc = push_hist(m, m->vel.d(), A);
     if (rc) return rc;
