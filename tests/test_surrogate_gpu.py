@@ -242,3 +242,49 @@ def test_failed_check_falls_back_to_the_direct_sum(need_gpu, developed, small_ne
     assert st["checks"] >= 4 and st["failed_checks"] == st["checks"]
     assert st["network_steps"] == 0 and st["direct_far_steps"] > 0
     assert 0.0 < st["last_err_nodes"] < 1.0
+
+
+def test_full_rotor_surrogate_within_one_percent(need_gpu):
+    """The benchmark's rotor (3 blades x 31 nodes, 5-degree steps, PCC,
+    cut-off core) with a 432-age wake: developed 8 revolutions in FP32, a
+    network trained on seeded synthetic wakes and on the march's own direct
+    sums, then 4 more revolutions with the surrogate (split at 144 ages,
+    near targets below 72, one check per revolution, 1% tolerance) against
+    the plain FP32 march from the same state: thrust, torque and power
+    within 1%, and the network serving the steps between checks."""
+    from winds import surrogate as S
+    from winds.march import March
+    from winds.mlp import MLP
+    o32 = winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, precision=winds.PREC_FP32C,
+                     d=1e-4)
+    per_rev = 72
+    rot = cases.tables(cases.synthetic_rotor(nb=3, nst=31, n_times=13 * per_rev))
+    m = March(rot, 432, o32, integrator="pcc")
+    m.step(6 * per_rev)
+    msamp = S.march_training_set(m, o32, steps=2 * per_rev, every=6, split=144, target_ages=72)
+    flush_print(f"[full rotor] developed to time index {m.info()['p']}, "
+                f"{msamp.F.shape[0]} march samples")
+    synth = S.build_training_set(200_000, o32, seed=7, split=144, target_ages=72, max_ages=432)
+    net = MLP(S.NIN, 256, 4, S.NOUT, 10240, seed=7)
+    in_norm, out_norm, secs, losses = S.train_two_phase(net, synth, msamp, steps=4000, batch=4096)
+    inf = m.info()
+    nodes, _, _ = m.state()
+    A, Ag = inf["ages"], inf["ages_solved"]
+    gs, gt = m.state_circulation(Ag)
+    state = (inf["p"], nodes, gs[:, :, :A - 1], gt[:, :, :A - 2])
+    del m
+    a, b = March(rot, 432, o32, integrator="pcc"), March(rot, 432, o32, integrator="pcc")
+    for f in (a, b):
+        f.set_state(*state)
+    b.set_surrogate(S.MarchSurrogate(mode=0, check_every=per_rev), net, in_norm, out_norm)
+    steps = 4 * per_rev
+    for f in (a, b):
+        f.step(steps)
+        f.sync()
+    err = rel_perf_err(b.perf(), a.perf())
+    st = b.surr_stats()
+    flush_print(f"[full rotor] training {sum(secs):.1f} s, losses {losses}; T/Q/P err {err:.2e}; "
+                f"{st}")
+    assert st["checks"] == 4 and st["network_steps"] + st["direct_far_steps"] == steps - 4
+    assert st["network_steps"] >= steps // 2
+    assert err <= 0.01
