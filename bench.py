@@ -325,7 +325,7 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
 
 
 def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train_steps: int,
-                check_every: int = 72, fp64_revs: int = 2):
+                check_every: int = 72, fp64_revs: int = 2, total_steps: int = 3600):
     """Free-wake time march of a 3-blade, 31-node rotor at 5 deg steps with
     PCC (2 wake evaluations per step), an FP64 Kutta-Joukowski solve and a
     1,440-age wake (~10 rotor diameters):
@@ -334,7 +334,9 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
     (2) step time at the full length from that state: FP32, FP64, surrogate
         (network), surrogate with the direct far sum instead of the network;
     (3) `revs` more revolutions from that state: FP32 and surrogate, and the
-        first fp64_revs of them in FP64 as the reference for both.
+        first fp64_revs of them in FP64 as the reference for both; the
+        surrogate march goes on to total_steps steps in all (the workload's
+        3,600), its later revolutions set beside the FP32 march's last ones.
     Returns (march report, surrogate report)."""
     import numpy as np
 
@@ -346,7 +348,9 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
     per_rev = 72
     steps = revs * per_rev
     dev_steps = dev_revs * per_rev
-    tb = cases.tables(cases.synthetic_rotor(nb=3, nst=31, n_times=dev_steps + steps + 16))
+    surr_steps = max(steps, total_steps - dev_steps) if surrogate else 0
+    tb = cases.tables(cases.synthetic_rotor(nb=3, nst=31,
+                                            n_times=dev_steps + max(steps, surr_steps) + 16))
     sc = S.MarchSurrogate(check_every=check_every)
     out = {"rotor": "3 blades x 31 nodes, R 63 m, TSR 7, U 11.4 m/s, 5 deg steps, PCC, "
                     "kinematics.m/velocity.m tables, BEM start",
@@ -442,7 +446,7 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
     hist = {}
     plan = [("fp32c", winds.PREC_FP32C, None, steps)]
     if surrogate:
-        plan += [("surrogate", winds.PREC_FP32C, 0, steps)]
+        plan += [("surrogate", winds.PREC_FP32C, 0, surr_steps)]
     plan += [("fp64", winds.PREC_FP64, None, fp64_revs * per_rev)]
     for name, prec, mode, nst in plan:
         t0 = time.perf_counter()
@@ -481,13 +485,26 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
         out["surrogate_vs_fp32c"] = errs(hist["surrogate"], hist["fp32c"], steps)
         log(f"surrogate vs fp64 over {fp64_revs} revs: {out['surrogate_vs_fp64']}; "
             f"vs fp32c over {revs} revs: {out['surrogate_vs_fp32c']}")
+        # the surrogate's last revolution against the FP32 march's last one
+        # (the rotor is periodic: same azimuths, same inflow)
+        hs, hf = hist["surrogate"], hist["fp32c"]
+        last_s, last_f = hs[-per_rev:], hf[-per_rev:]
+        out["surrogate_last_rev_vs_fp32c_last_rev"] = {
+            "surrogate_revs": int(surr_steps // per_rev), "fp32c_revs": int(revs),
+            "thrust_mean_rel": float(abs(np.mean(last_s[:, 1]) - np.mean(last_f[:, 1]))
+                                     / abs(np.mean(last_f[:, 1]))),
+            "power_mean_rel": float(abs(np.mean(last_s[:, 3]) - np.mean(last_f[:, 3]))
+                                    / abs(np.mean(last_f[:, 3])))}
+        log(f"surrogate revolution {surr_steps // per_rev} vs fp32c revolution {revs}: "
+            f"{out['surrogate_last_rev_vs_fp32c_last_rev']}")
     out.update({"revs": revs, "steps": steps,
                 "thrust_N_mean_fp32c": float(np.mean(hist["fp32c"][:, 1])),
                 "power_W_mean_fp32c": float(np.mean(hist["fp32c"][:, 3])),
                 "config5_steps_covered": (f"{dev_steps + steps} of 3600 steps in FP32 "
                                           f"({dev_steps} cold start + {steps} continued), "
-                                          f"{steps} with the surrogate, {n64} in FP64 from the "
-                                          f"developed 1,440-age state")})
+                                          f"{dev_steps + surr_steps} with the surrogate "
+                                          f"({dev_steps} FP32 cold start + {surr_steps}), "
+                                          f"{n64} in FP64 from the developed 1,440-age state")})
     return out, surr_rep
 
 
