@@ -168,3 +168,35 @@ def test_graph_replayed_steps_match_eager_steps(need_gpu):
         assert np.array_equal(a.get_params(), b.get_params()), it
         # the loss is an atomic sum over workgroups: equal to rounding
         assert abs(float(la.item()) - float(lb.item())) <= 1e-6 * float(lb.item()), it
+
+
+@pytest.mark.parametrize("shape", [(49, 256, 4, 3), (200, 128, 2, 5), (7, 64, 1, 32),
+                                   (256, 96, 3, 1)])
+def test_fused_forward_equals_the_per_layer_forward(need_gpu, shape):
+    """winds_mlp_forward in one kernel per 64 rows (activations in LDS) and
+    as one GEMM per layer: the same MFMA sequence and epilogues, so bitwise
+    the same outputs, for ragged batches and widths that are not multiples
+    of the 128-k slices."""
+    import torch
+    from winds.mlp import MLP
+    n_in, hidden, layers, n_out = shape
+    net = MLP(n_in, hidden, layers, n_out, 10240, seed=11)
+    rng = np.random.default_rng(5)
+    flat = net.get_params()
+    net.set_params((flat + 0.05 * rng.standard_normal(flat.shape)).astype(np.float32))
+    for B in (1, 63, 64, 65, 1000, 6786, 10240):
+        X = torch.from_numpy(rng.standard_normal((B, n_in)).astype(np.float32)).cuda()
+        net.set_fused(True)
+        Yf = net.forward(X).cpu().numpy()
+        net.set_fused(False)
+        Yl = net.forward(X).cpu().numpy()
+        assert np.array_equal(Yf.view(np.uint32), Yl.view(np.uint32)), (shape, B)
+    net.set_fused(True)
+
+
+def test_fused_forward_refuses_wide_layers(need_gpu):
+    from winds.mlp import MLP
+    net = MLP(16, 512, 1, 2, 128, seed=1)
+    with pytest.raises(Exception):
+        net.set_fused(True)
+    net.set_fused(False)

@@ -11,7 +11,8 @@
 // SiLU's derivative (backward) and writes the BF16 activations or deltas in
 // both [batch][feature] and [feature][batch] layouts, so that the forward
 // pass (X W^T), the input gradient (dZ W) and the weight gradient (dZ^T X)
-// all use this one kernel.
+// all use this one kernel.  Inference runs the whole network in one kernel
+// per 32 rows instead (mlp_forward_fused), with bitwise the same results.
 
 #include <hip/hip_runtime.h>
 
@@ -31,19 +32,19 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef uint16_t bf16;
 
-// round to nearest even
-__device__ __forceinline__ bf16 to_bf16(float x) {
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (bf16)(u >> 16);  // inf / nan
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (bf16)(u >> 16);
-}
+// round to nearest even (gfx950's v_cvt_pk_bf16_f32)
+__device__ __forceinline__ bf16 to_bf16(float x) { return __builtin_bit_cast(bf16, (__bf16)x); }
 __device__ __forceinline__ float from_bf16(bf16 h) { return __uint_as_float((uint32_t)h << 16); }
 
 typedef short bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + expf(-z)); }
+// logistic function with the hardware exponential and reciprocal (~1 ulp
+// each): the epilogues evaluate it for every activation, and the accurate
+// library forms cost ~4x the instructions there
+__device__ __forceinline__ float sigm(float z) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+}
 
 // What the GEMM epilogue does with each element x of the product.
 enum Epi : int {
@@ -183,6 +184,139 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16 *__restrict__ A, int
                 *(bf16x8 *)(e.AT + (int64_t)(col0 + c) * e.ldt + row0 + r8) = q;
             }
         }
+    }
+}
+
+// The whole forward pass for 32 rows in one workgroup (inference): the
+// rows' activations stay in LDS between layers (BF16, [32][K], K up to
+// 256), each wave computes its 32-column block of the next layer from them
+// with the weights' fragments read straight from device memory (the waves
+// use different weight rows, so staging them would share nothing), and the
+// epilogue writes bias + SiLU in BF16 back to LDS; the output layer writes
+// FP32 rows of n_out.  Eight waves, one 32 x 32 block each: a small tile
+// spreads a batch over many CUs, which at these sizes bounds the time
+// (each layer's MFMAs and SiLU epilogue run on the CU that holds the rows).  The MFMA sequence of every block -- 16-k steps in
+// increasing k over K rounded up to the 128-k slices of gemm_bf16, zero
+// operands past the padded width -- and the epilogue arithmetic are those
+// of gemm_bf16 with pack_input and take_cols, so the result is bitwise the
+// per-layer forward's.  512 threads; hidden widths up to 256.
+constexpr int kFRows = 32;
+constexpr int kFMaxK = 256;
+constexpr int kFLdsRow = kFMaxK + 8;
+
+struct FusedLayer {
+    const bf16 *Wb;      // [Np][Kp]
+    const float *bias;   // [Np]
+    int Kp, Np;
+};
+constexpr int kFMaxLayers = 17;
+struct FusedNet {
+    FusedLayer L[kFMaxLayers];
+    int nl, n_in, n_out;
+};
+
+// Wave w (of 8) computes column block w of layer li when it exists (the
+// output layer has one).
+__device__ __forceinline__ bool fused_active(const FusedNet &net, int li, int w) {
+    return w < net.L[li].Np / 32;
+}
+
+constexpr int kFSteps = kFMaxK / 16;  // 16-k steps of the widest layer
+
+constexpr int kFThreads = 512;
+__global__ __launch_bounds__(kFThreads) void mlp_forward_fused(FusedNet net, const float *__restrict__ X,
+                                                         int B, float *__restrict__ Y) {
+    __shared__ __attribute__((aligned(16))) bf16 act[2][kFRows][kFLdsRow];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int row0 = blockIdx.x * kFRows;
+    const bf16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    // a layer's weight fragments for this wave, all requested at once (one
+    // device-memory latency per layer, overlapped with the previous layer's
+    // epilogue)
+    bf16x8 bw[kFSteps];
+    auto load_w = [&](int li) {
+        const FusedLayer L = net.L[li];
+        const bool on = fused_active(net, li, w);
+        const int K = (L.Kp + kSlice - 1) / kSlice * kSlice;
+        const bf16 *row = L.Wb + (int64_t)(w * 32 + (l & 31)) * L.Kp;
+#pragma unroll
+        for (int s = 0; s < kFSteps; ++s) {
+            const int k = 16 * s + 8 * (l >> 5);
+            bw[s] = (16 * s < K && on && k < L.Kp) ? *(const bf16x8 *)(row + k) : zero;
+        }
+    };
+    load_w(0);
+    // the input rows in BF16 (they are contiguous in X: eight loads per
+    // thread in flight at a time), zero past n_in up to the first layer's
+    // slices and in the rows past B
+    const int K0 = (net.L[0].Kp + kSlice - 1) / kSlice * kSlice;
+    const int nrows = B - row0 < kFRows ? B - row0 : kFRows;
+    const int nflat = nrows * net.n_in;
+    const float *Xr = X + (int64_t)row0 * net.n_in;
+    for (int base = 0; base < nflat; base += kFThreads * 8) {
+        float v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int f = base + t + kFThreads * j;
+            v[j] = f < nflat ? Xr[f] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int f = base + t + kFThreads * j;
+            if (f < nflat) {
+                const int r = f / net.n_in;
+                act[0][r][f - r * net.n_in] = to_bf16(v[j]);
+            }
+        }
+    }
+    for (int i = t; i < kFRows * K0; i += kFThreads) {
+        const int r = i / K0, k = i - r * K0;
+        if (r >= nrows || k >= net.n_in) act[0][r][k] = 0;
+    }
+    __syncthreads();
+    for (int li = 0; li < net.nl; ++li) {
+        const FusedLayer L = net.L[li];
+        const bool last = li + 1 == net.nl;
+        const int K = (L.Kp + kSlice - 1) / kSlice * kSlice;
+        const bf16(*in)[kFLdsRow] = act[li & 1];
+        bf16(*out)[kFLdsRow] = act[(li + 1) & 1];
+        const bool on = fused_active(net, li, w);
+        f32x16 acc = {0};
+        if (on) {
+#pragma unroll
+            for (int s = 0; s < kFSteps; ++s) {
+                if (16 * s >= K) break;
+                const int k = 16 * s + 8 * (l >> 5);
+                const bf16x8 a = *(const bf16x8 *)&in[l & 31][k];
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    __builtin_bit_cast(bf16x8v, a), __builtin_bit_cast(bf16x8v, bw[s]), acc, 0, 0,
+                    0);
+            }
+        }
+        if (!last) load_w(li + 1);
+        if (on) {
+            const int col = w * 32 + (l & 31);
+            const float bias = L.bias[col];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int rl = 8 * (v >> 2) + 4 * (l >> 5) + (v & 3);
+                const float z = acc[v] + bias;
+                if (!last) {
+                    out[rl][col] = to_bf16(z * sigm(z));
+                } else if (col < net.n_out && row0 + rl < B) {
+                    Y[(int64_t)(row0 + rl) * net.n_out + col] = z;
+                }
+            }
+        }
+        // the next layer's slices past this layer's width are zeros
+        if (!last) {
+            const int Kn = (L.Np + kSlice - 1) / kSlice * kSlice;
+            for (int i = t; i < kFRows * (Kn - L.Np); i += kFThreads) {
+                const int r = i / (Kn - L.Np), k = L.Np + i % (Kn - L.Np);
+                out[r][k] = 0;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -437,6 +571,8 @@ struct winds_mlp {
     int repeats = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // inference through mlp_forward_fused (hidden widths up to 256)
+    bool fused = false;
 };
 
 namespace {
@@ -655,6 +791,7 @@ int winds_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_hidden_layers, in
     int rc = refresh(m, m->st);
     if (!rc && hipStreamSynchronize(m->st) != hipSuccess) rc = set_error(WINDS_E_HIP, "sync");
     if (rc) return fail(rc);
+    m->fused = n_hidden <= kFMaxK && m->nl <= kFMaxLayers;
     *out = m;
     return WINDS_OK;
 }
@@ -717,12 +854,34 @@ int winds_mlp_forward(winds_mlp *m, const float *X, int32_t B, float *Y, void *s
     if (B == 0) return WINDS_OK;
     if (!X || !Y) return set_error(WINDS_E_ARG, "null input/output");
     hipStream_t st = (hipStream_t)stream;  // NULL: the default stream
+    if (m->fused) {
+        FusedNet net{};
+        net.nl = m->nl;
+        net.n_in = m->n_in;
+        net.n_out = m->n_out;
+        for (int i = 0; i < m->nl; ++i)
+            net.L[i] = FusedLayer{m->L[i].Wb.as<bf16>(), m->L[i].b.as<float>(), m->L[i].Kp,
+                                  m->L[i].Np};
+        mlp_forward_fused<<<(unsigned)((B + kFRows - 1) / kFRows), kFThreads, 0, st>>>(net, X, B,
+                                                                                     Y);
+        MLP_HIP(hipGetLastError());
+        return WINDS_OK;
+    }
     const int Bp = rup(B, 64);
     int rc = forward(m, X, B, Bp, false, st);
     if (rc) return rc;
     const Layer &lo = m->L[m->nl - 1];
     take_cols<<<nblk((int64_t)B * m->n_out), 256, 0, st>>>(lo.Z.as<float>(), lo.Np, B, m->n_out, Y);
     MLP_HIP(hipGetLastError());
+    return WINDS_OK;
+}
+
+int winds_mlp_set_fused(winds_mlp *m, int32_t on) {
+    winds::clear_error();
+    if (!m) return set_error(WINDS_E_ARG, "null network");
+    if (on && !(m->hidden <= kFMaxK && m->nl <= kFMaxLayers && m->L[0].Kp <= kFMaxK))
+        return set_error(WINDS_E_DIMS, "fused forward needs hidden width and inputs up to 256");
+    m->fused = on != 0;
     return WINDS_OK;
 }
 

@@ -28,7 +28,9 @@ def _bind(L):
     L.winds_mlp_forward.argtypes = [P, P, C.c_int32, P, P]
     L.winds_mlp_gradients.argtypes = [P, P, P, C.c_int32, F, F]
     L.winds_mlp_train_step.argtypes = [P, P, P, C.c_int32, C.c_float, P, P]
-    for f in ("create", "get_params", "set_params", "forward", "gradients", "train_step"):
+    L.winds_mlp_set_fused.argtypes = [P, C.c_int32]
+    for f in ("create", "get_params", "set_params", "forward", "gradients", "train_step",
+              "set_fused"):
         getattr(L, "winds_mlp_" + f).restype = C.c_int
     L._winds_mlp_bound = True
     return L
@@ -114,6 +116,11 @@ class MLP:
         _lib.check(self.L.winds_mlp_forward(self._m, C.c_void_p(X.data_ptr()), B,
                                             C.c_void_p(out.data_ptr()), C.c_void_p(st)))
         return out
+
+    def set_fused(self, on: bool) -> None:
+        """Inference in one kernel per 64 rows (True, the default for hidden
+        widths up to 256) or one GEMM per layer (False); the same results."""
+        _lib.check(self.L.winds_mlp_set_fused(self._m, 1 if on else 0))
 
     def gradients(self, X, T):
         """(flat FP32 gradient of the MSE, loss) for one batch, without an update."""
