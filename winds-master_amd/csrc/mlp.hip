@@ -461,34 +461,35 @@ __global__ void zero_scalar(float *__restrict__ p) {
 // Adam on one layer: weights i < N*K (then their BF16 copies W [N][K] and
 // W^T [K][N]), biases after them; gradients as partial slabs (gW [S][N][K],
 // gb [Sb][N]); hyp = {lr, c1, c2}
-__global__ void adam_layer(float *__restrict__ W, const float *__restrict__ gW,
-                           float *__restrict__ mW, float *__restrict__ vW, float *__restrict__ b,
-                           const float *__restrict__ gb, float *__restrict__ mb,
-                           float *__restrict__ vb, int N, int K, int S, int Sb,
-                           const float *__restrict__ hyp, float b1, float b2, float eps,
-                           bf16 *__restrict__ Wb, bf16 *__restrict__ WT) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const float lr = hyp[0], c1 = hyp[1], c2 = hyp[2];
-    const int64_t nw = (int64_t)N * K;
-    if (i >= nw + N) return;
-    float *p = i < nw ? W : b;
-    const float *g = i < nw ? gW : gb;
-    float *m = i < nw ? mW : mb;
-    float *v = i < nw ? vW : vb;
+// Adam on element i of one layer's weights (i < N K) or biases, with the
+// gradient summed from its partials in order (S split slabs of the weights,
+// Sb row-block partials of the biases).
+struct AdamLayer {
+    float *W, *mW, *vW, *b, *mb, *vb;
+    const float *gW, *gb;
+    bf16 *Wb, *WT;
+    int N, K, S, Sb;
+    int64_t off;  // first element of this layer in the flat launch
+};
+__device__ __forceinline__ void adam_elem(const AdamLayer &L, int64_t i, float lr, float c1,
+                                          float c2, float b1, float b2, float eps) {
+    const int64_t nw = (int64_t)L.N * L.K;
+    float *p = i < nw ? L.W : L.b;
+    const float *g = i < nw ? L.gW : L.gb;
+    float *m = i < nw ? L.mW : L.mb;
+    float *v = i < nw ? L.vW : L.vb;
     const int64_t j = i < nw ? i : i - nw;
-    // the gradient: the sum of the partials (S split slabs of the weights,
-    // Sb 64-row blocks of the biases), in order
-    const int64_t stride = i < nw ? nw : N;
-    const int np = i < nw ? S : Sb;
+    const int64_t stride = i < nw ? nw : L.N;
+    const int np = i < nw ? L.S : L.Sb;
     float gi = 0.0f;
     for (int z0 = 0; z0 < np; z0 += 16) {
         // sixteen loads in flight, then their sum in order
-        float v[16];
+        float u[16];
 #pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = z0 + q < np ? g[(int64_t)(z0 + q) * stride + j] : 0.0f;
+        for (int q = 0; q < 16; ++q) u[q] = z0 + q < np ? g[(int64_t)(z0 + q) * stride + j] : 0.0f;
 #pragma unroll
         for (int q = 0; q < 16; ++q)
-            if (z0 + q < np) gi = z0 + q == 0 ? v[q] : gi + v[q];
+            if (z0 + q < np) gi = z0 + q == 0 ? u[q] : gi + u[q];
     }
     const float mi = b1 * m[j] + (1.0f - b1) * gi;
     const float vi = b2 * v[j] + (1.0f - b2) * gi * gi;
@@ -497,11 +498,28 @@ __global__ void adam_layer(float *__restrict__ W, const float *__restrict__ gW,
     const float x = p[j] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     p[j] = x;
     if (i < nw) {
-        const int n = (int)(i / K), k = (int)(i - (int64_t)n * K);
+        const int n = (int)(i / L.K), k = (int)(i - (int64_t)n * L.K);
         const bf16 h = to_bf16(x);
-        Wb[i] = h;
-        WT[(int64_t)k * N + n] = h;
+        L.Wb[i] = h;
+        L.WT[(int64_t)k * L.N + n] = h;
     }
+}
+
+// One Adam step over every layer in one launch: element e of the flat
+// range belongs to the last layer whose off <= e.
+constexpr int kMaxAdamLayers = 17;
+struct AdamNet {
+    AdamLayer L[kMaxAdamLayers];
+    int nl;
+    int64_t total;
+};
+__global__ void adam_all(AdamNet net, const float *__restrict__ hyp, float b1, float b2,
+                         float eps) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= net.total) return;
+    int li = 0;
+    while (li + 1 < net.nl && net.L[li + 1].off <= e) ++li;
+    adam_elem(net.L[li], e - net.L[li].off, hyp[0], hyp[1], hyp[2], b1, b2, eps);
 }
 
 // FP32 master W [N][K] -> BF16 W [N][K] and W^T [K][N]
@@ -674,14 +692,20 @@ int train_body(winds_mlp *m, const float *X, const float *T, int B, hipStream_t 
     int rc = forward(m, X, B, Bp, true, st);
     if (!rc) rc = backward(m, T, B, Bp, st, m->loss.as<float>());
     if (rc) return rc;
-    for (Layer &l : m->L) {
-        const int64_t nw = (int64_t)l.Np * l.Kp;
-        adam_layer<<<nblk(nw + l.Np), 256, 0, st>>>(
-            l.W.as<float>(), l.gW.as<float>(), l.mW.as<float>(), l.vW.as<float>(), l.b.as<float>(),
-            l.gb.as<float>(), l.mb.as<float>(), l.vb.as<float>(), l.Np, l.Kp, splits_of(Bp), Bp / 64,
-            m->hyp.as<const float>(), 0.9f, 0.999f, 1e-8f, l.Wb.as<bf16>(), l.WT.as<bf16>());
-        MLP_HIP(hipGetLastError());
+    AdamNet an{};
+    an.nl = m->nl;
+    int64_t off = 0;
+    for (int i = 0; i < m->nl; ++i) {
+        Layer &l = m->L[i];
+        an.L[i] = AdamLayer{l.W.as<float>(), l.mW.as<float>(), l.vW.as<float>(), l.b.as<float>(),
+                            l.mb.as<float>(), l.vb.as<float>(), l.gW.as<float>(), l.gb.as<float>(),
+                            l.Wb.as<bf16>(), l.WT.as<bf16>(), l.Np, l.Kp, splits_of(Bp), Bp / 64,
+                            off};
+        off += (int64_t)l.Np * l.Kp + l.Np;
     }
+    an.total = off;
+    adam_all<<<nblk(off), 256, 0, st>>>(an, m->hyp.as<const float>(), 0.9f, 0.999f, 1e-8f);
+    MLP_HIP(hipGetLastError());
     return WINDS_OK;
 }
 
