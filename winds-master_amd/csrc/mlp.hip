@@ -76,15 +76,15 @@ constexpr int kMaxSplit = 16;        // K splits of the weight-gradient product
 // zeros and their results dropped.  256 threads.  Split K (kEpiF32 only):
 // workgroup z sums k in [z kspan, (z+1) kspan) into C + z slab.
 template <int EPI>
-__global__ __launch_bounds__(256) void gemm_bf16(const bf16 *__restrict__ A, int lda,
-                                                 const bf16 *__restrict__ B, int ldb, int M,
-                                                 int N, int K, int kspan, int64_t slab,
-                                                 EpiArgs e) {
+__device__ __forceinline__ void gemm_block(const bf16 *__restrict__ A, int lda,
+                                           const bf16 *__restrict__ B, int ldb, int M, int N,
+                                           int K, int kspan, int64_t slab, EpiArgs e, int bx,
+                                           int by, int bz) {
     __shared__ __attribute__((aligned(16))) bf16 sA[2][kTile][kLdsRow];
     __shared__ __attribute__((aligned(16))) bf16 sB[2][kTile][kLdsRow];
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
-    const int row0 = blockIdx.y * kTile, col0 = blockIdx.x * kTile;
-    const int kb = blockIdx.z * kspan, ke = min(K, kb + kspan);
+    const int row0 = by * kTile, col0 = bx * kTile;
+    const int kb = bz * kspan, ke = min(K, kb + kspan);
     // loader: a 64 x 128 slice of each operand is 1024 pieces of 8 BF16, four
     // per thread (piece idx: row idx / 16, k 8 (idx % 16)); K and the split
     // bounds are multiples of 32, so a piece is wholly inside or outside
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16 *__restrict__ A, int
         __syncthreads();
         buf ^= 1;
     }
-    if constexpr (EPI == kEpiF32) e.C += blockIdx.z * slab;
+    if constexpr (EPI == kEpiF32) e.C += bz * slab;
     const int col = col0 + wc + (l & 31);
     if constexpr (EPI == kEpiF32 || EPI == kEpiBias) {
         if (col >= N) return;
@@ -185,6 +185,39 @@ __global__ __launch_bounds__(256) void gemm_bf16(const bf16 *__restrict__ A, int
             }
         }
     }
+}
+
+template <int EPI>
+__global__ __launch_bounds__(256) void gemm_bf16(const bf16 *__restrict__ A, int lda,
+                                                 const bf16 *__restrict__ B, int ldb, int M,
+                                                 int N, int K, int kspan, int64_t slab,
+                                                 EpiArgs e) {
+    gemm_block<EPI>(A, lda, B, ldb, M, N, K, kspan, slab, e, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Several FP32-epilogue products in one launch (the layers' weight
+// gradients): workgroup g belongs to the last problem whose first <= g and
+// takes that problem's block (g - first) in x-fastest order.
+struct GemmProblem {
+    const bf16 *A, *B;
+    float *C;
+    int lda, ldb, ldc, M, N, K, kspan, gx, gy;
+    int64_t slab;
+    int first;
+};
+constexpr int kMaxProblems = 17;
+struct GemmGroup {
+    GemmProblem P[kMaxProblems];
+    int n;
+};
+__global__ __launch_bounds__(256) void gemm_bf16_group(GemmGroup g) {
+    int pi = 0;
+    while (pi + 1 < g.n && g.P[pi + 1].first <= (int)blockIdx.x) ++pi;
+    const GemmProblem &P = g.P[pi];
+    const int q = (int)blockIdx.x - P.first;
+    const int bx = q % P.gx, by = (q / P.gx) % P.gy, bz = q / (P.gx * P.gy);
+    EpiArgs e{P.C, P.ldc, nullptr, nullptr, 0, nullptr, 0, nullptr, 0};
+    gemm_block<kEpiF32>(P.A, P.lda, P.B, P.ldb, P.M, P.N, P.K, P.kspan, P.slab, e, bx, by, bz);
 }
 
 // The whole forward pass for 32 rows in one workgroup (inference): the
@@ -423,10 +456,10 @@ __global__ __launch_bounds__(256) void tile_pack(float *__restrict__ Z, const fl
 // Partial column sums of dZ [Bp][N] over blocks of 64 rows: workgroup
 // (x, z) sums rows 64z..64z+63 of columns 64x..64x+63 into db [z][N]; each
 // thread adds 16 rows in order, the four quarters are then added in order
-__global__ __launch_bounds__(256) void col_sum_part(const float *__restrict__ dZ, int Bp, int N,
-                                                    float *__restrict__ db) {
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63), q = threadIdx.x >> 6;
-    const int r0 = blockIdx.y * 64 + 16 * q;
+__device__ __forceinline__ void col_sum_block(const float *__restrict__ dZ, int N,
+                                              float *__restrict__ db, int bx, int by) {
+    const int c = bx * 64 + (threadIdx.x & 63), q = threadIdx.x >> 6;
+    const int r0 = by * 64 + 16 * q;
     float s = 0.0f;
     if (c < N) {
         float v[16];
@@ -440,8 +473,27 @@ __global__ __launch_bounds__(256) void col_sum_part(const float *__restrict__ dZ
     __syncthreads();
     if (q == 0 && c < N) {
         const int j = threadIdx.x & 63;
-        db[(int64_t)blockIdx.y * N + c] = ((red[0][j] + red[1][j]) + red[2][j]) + red[3][j];
+        db[(int64_t)by * N + c] = ((red[0][j] + red[1][j]) + red[2][j]) + red[3][j];
     }
+}
+
+// Every layer's bias-gradient partials in one launch: workgroup g belongs
+// to the last layer whose first <= g.
+struct ColSumLayer {
+    const float *dZ;
+    float *db;
+    int N, gx, first;
+};
+struct ColSumGroup {
+    ColSumLayer L[kMaxProblems];
+    int n;
+};
+__global__ __launch_bounds__(256) void col_sum_group(ColSumGroup g) {
+    int li = 0;
+    while (li + 1 < g.n && g.L[li + 1].first <= (int)blockIdx.x) ++li;
+    const ColSumLayer &L = g.L[li];
+    const int q = (int)blockIdx.x - L.first;
+    col_sum_block(L.dZ, L.N, L.db, q % L.gx, q / L.gx);
 }
 
 // Adam's per-step scalars {learning rate, 1 - b1^t, 1 - b2^t} into device
@@ -658,28 +710,40 @@ int backward(winds_mlp *m, const float *T, int B, int Bp, hipStream_t st, float 
         lo.dZ.as<float>(), nullptr, nullptr, Bp, lo.Np, Bp, lo.Np, lo.Db.as<bf16>(),
         lo.DT.as<bf16>());
     MLP_HIP(hipGetLastError());
-    for (int i = m->nl - 1; i >= 0; --i) {
+    // the deltas, layer by layer: dZ_prev [Bp][Kp] = (dZ [Bp][Np] * W [Np][Kp])
+    // SiLU'(Z_prev) (B operand W^T [Kp][Np])
+    for (int i = m->nl - 1; i > 0; --i) {
+        Layer &l = m->L[i];
+        Layer &p = m->L[i - 1];
+        const EpiArgs e{p.dZ.as<float>(), p.Np, nullptr, p.Z.as<float>(), p.Np,
+                        p.Db.as<bf16>(), p.Np, p.DT.as<bf16>(), Bp};
+        gemm<kEpiSiluBack>(st, l.Db.as<bf16>(), l.Np, l.WT.as<bf16>(), l.Np, Bp, l.Kp, l.Np, e);
+        MLP_HIP(hipGetLastError());
+    }
+    // then every layer's weight gradient in one launch, dW [Np][Kp] = dZ^T
+    // [Np][Bp] * X^T [Kp][Bp]^T as S partial slabs over the batch
+    // (S = splits_of(Bp)), and the bias gradients as one partial per 64 rows
+    const int S = splits_of(Bp);
+    GemmGroup gg{};
+    ColSumGroup cg{};
+    gg.n = cg.n = m->nl;
+    int gfirst = 0, cfirst = 0;
+    for (int i = 0; i < m->nl; ++i) {
         Layer &l = m->L[i];
         const bf16 *xt = i == 0 ? m->XT.as<bf16>() : m->L[i - 1].AT.as<bf16>();
-        // dW [Np][Kp] = dZ^T [Np][Bp] * X^T [Kp][Bp]^T, as S partial slabs
-        // over the batch (S = splits_of(Bp)); db as one partial per 64 rows
-        const int S = splits_of(Bp);
-        gemm<kEpiF32>(st, l.DT.as<bf16>(), Bp, xt, Bp, l.Np, l.Kp, Bp,
-                      epi_f32(l.gW.as<float>(), l.Kp), S, (int64_t)l.Np * l.Kp);
-        col_sum_part<<<dim3((l.Np + 63) / 64, Bp / 64), 256, 0, st>>>(l.dZ.as<float>(), Bp, l.Np,
-                                                                      l.gb.as<float>());
-        MLP_HIP(hipGetLastError());
-        if (i > 0) {
-            Layer &p = m->L[i - 1];
-            // dZ_prev [Bp][Kp] = (dZ [Bp][Np] * W [Np][Kp]) SiLU'(Z_prev)
-            // (B operand W^T [Kp][Np])
-            const EpiArgs e{p.dZ.as<float>(), p.Np, nullptr, p.Z.as<float>(), p.Np,
-                            p.Db.as<bf16>(), p.Np, p.DT.as<bf16>(), Bp};
-            gemm<kEpiSiluBack>(st, l.Db.as<bf16>(), l.Np, l.WT.as<bf16>(), l.Np, Bp, l.Kp, l.Np,
-                               e);
-            MLP_HIP(hipGetLastError());
-        }
+        const int kspan = S > 1 ? rup((Bp + S - 1) / S, kSlice) : Bp;
+        const int gx = (l.Kp + kTile - 1) / kTile, gy = (l.Np + kTile - 1) / kTile;
+        gg.P[i] = GemmProblem{l.DT.as<bf16>(), xt, l.gW.as<float>(), Bp, Bp, l.Kp, l.Np, l.Kp,
+                              Bp, kspan, gx, gy, (int64_t)l.Np * l.Kp, gfirst};
+        gfirst += gx * gy * S;
+        const int cx = (l.Np + 63) / 64;
+        cg.L[i] = ColSumLayer{l.dZ.as<float>(), l.gb.as<float>(), l.Np, cx, cfirst};
+        cfirst += cx * (Bp / 64);
     }
+    gemm_bf16_group<<<gfirst, 256, 0, st>>>(gg);
+    MLP_HIP(hipGetLastError());
+    col_sum_group<<<cfirst, 256, 0, st>>>(cg);
+    MLP_HIP(hipGetLastError());
     return WINDS_OK;
 }
 
