@@ -43,12 +43,12 @@ int winds_mlp_set_params(winds_mlp *m, const float *host);
  * they must not run concurrently on different streams. */
 int winds_mlp_forward(winds_mlp *m, const float *X, int32_t B, float *Y, void *stream);
 
-/* Inference path of winds_mlp_forward (no counterpart in the reference,
- * which has no network): on = 1 runs the whole pass in one kernel per 64
- * rows, activations held in LDS (the default where the hidden width is at
- * most 256); on = 0 runs one GEMM per layer, the kernels training uses.
- * Both give bitwise the same Y.  WINDS_E_DIMS for on = 1 with a hidden
- * width above 256. */
+/* Forward path of winds_mlp_forward and of the training step (no
+ * counterpart in the reference, which has no network): on = 1 runs the
+ * whole forward pass in one kernel per 32 rows, activations held in LDS
+ * (the default where the hidden width is at most 256); on = 0 runs one GEMM
+ * per layer.  Both give bitwise the same outputs, gradients and updates.
+ * WINDS_E_DIMS for on = 1 with a hidden width above 256. */
 int winds_mlp_set_fused(winds_mlp *m, int32_t on);
 
 /* Mean-squared-error gradients of all parameters for one batch (no update),

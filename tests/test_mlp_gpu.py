@@ -200,3 +200,26 @@ def test_fused_forward_refuses_wide_layers(need_gpu):
     with pytest.raises(Exception):
         net.set_fused(True)
     net.set_fused(False)
+
+
+def test_fused_training_forward_equals_the_per_layer_forward(need_gpu):
+    """Training steps whose forward pass runs in one kernel per 32 rows and
+    steps whose forward runs one GEMM per layer: bitwise the same weights
+    after every step, and the same losses."""
+    import torch
+    from winds.mlp import MLP
+    rng = np.random.default_rng(9)
+    a = MLP(49, 256, 4, 3, 4096, seed=3)
+    b = MLP(49, 256, 4, 3, 4096, seed=3)
+    b.set_fused(False)
+    for B in (4096, 1000, 33):
+        X = torch.from_numpy(rng.standard_normal((B, 49)).astype(np.float32)).cuda()
+        T = torch.from_numpy(0.3 * rng.standard_normal((B, 3)).astype(np.float32)).cuda()
+        la, lb = torch.zeros(1, device="cuda"), torch.zeros(1, device="cuda")
+        for _ in range(3):
+            a.train_step(X, T, 1e-3, la)
+            b.train_step(X, T, 1e-3, lb)
+        torch.cuda.synchronize()
+        assert np.array_equal(a.get_params(), b.get_params()), B
+        # the loss is a sum of workgroup partials added atomically
+        assert abs(float(la.item()) - float(lb.item())) <= 1e-6 * abs(float(lb.item())), B

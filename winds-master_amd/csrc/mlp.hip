@@ -247,6 +247,16 @@ struct FusedNet {
     FusedLayer L[kFMaxLayers];
     int nl, n_in, n_out;
 };
+// What the training forward also leaves for the backward pass, as the
+// per-layer GEMMs do: each layer's FP32 pre-activations Z [Bp][Np], the
+// hidden layers' BF16 activations transposed AT [Np][Bp] and the input
+// transposed XT [Kp][Bp].
+struct FusedTrain {
+    float *Z[kFMaxLayers];
+    bf16 *AT[kFMaxLayers];
+    bf16 *XT;
+    int Bp;
+};
 
 // Wave w (of 8) computes column block w of layer li when it exists (the
 // output layer has one).
@@ -257,8 +267,24 @@ __device__ __forceinline__ bool fused_active(const FusedNet &net, int li, int w)
 constexpr int kFSteps = kFMaxK / 16;  // 16-k steps of the widest layer
 
 constexpr int kFThreads = 512;
+
+// rows [32] x columns [n] of an LDS tile -> out [n][ld] at column row0,
+// 16-byte stores of eight rows
+__device__ __forceinline__ void store_transposed(const bf16 (*tile)[kFLdsRow], int n, bf16 *out,
+                                                 int ld, int row0) {
+    for (int i = threadIdx.x; i < n * (kFRows / 8); i += kFThreads) {
+        const int c = i / (kFRows / 8), r8 = (i % (kFRows / 8)) * 8;
+        bf16x8 q;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) q[j] = (short)tile[r8 + j][c];
+        *(bf16x8 *)(out + (int64_t)c * ld + row0 + r8) = q;
+    }
+}
+
+template <bool TRAIN>
 __global__ __launch_bounds__(kFThreads) void mlp_forward_fused(FusedNet net, const float *__restrict__ X,
-                                                         int B, float *__restrict__ Y) {
+                                                         int B, float *__restrict__ Y,
+                                                         FusedTrain tr) {
     __shared__ __attribute__((aligned(16))) bf16 act[2][kFRows][kFLdsRow];
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int row0 = blockIdx.x * kFRows;
@@ -307,6 +333,7 @@ __global__ __launch_bounds__(kFThreads) void mlp_forward_fused(FusedNet net, con
         if (r >= nrows || k >= net.n_in) act[0][r][k] = 0;
     }
     __syncthreads();
+    if constexpr (TRAIN) store_transposed(act[0], net.L[0].Kp, tr.XT, tr.Bp, row0);
     for (int li = 0; li < net.nl; ++li) {
         const FusedLayer L = net.L[li];
         const bool last = li + 1 == net.nl;
@@ -334,9 +361,10 @@ __global__ __launch_bounds__(kFThreads) void mlp_forward_fused(FusedNet net, con
             for (int v = 0; v < 16; ++v) {
                 const int rl = 8 * (v >> 2) + 4 * (l >> 5) + (v & 3);
                 const float z = acc[v] + bias;
+                if constexpr (TRAIN) tr.Z[li][(int64_t)(row0 + rl) * L.Np + col] = z;
                 if (!last) {
                     out[rl][col] = to_bf16(z * sigm(z));
-                } else if (col < net.n_out && row0 + rl < B) {
+                } else if (!TRAIN && col < net.n_out && row0 + rl < B) {
                     Y[(int64_t)(row0 + rl) * net.n_out + col] = z;
                 }
             }
@@ -350,6 +378,8 @@ __global__ __launch_bounds__(kFThreads) void mlp_forward_fused(FusedNet net, con
             }
         }
         __syncthreads();
+        if constexpr (TRAIN)
+            if (!last) store_transposed(out, L.Np, tr.AT[li], tr.Bp, row0);
     }
 }
 
@@ -674,6 +704,26 @@ EpiArgs epi_f32(float *C, int ldc) { return EpiArgs{C, ldc, nullptr, nullptr, 0,
 // layouts and the pre-activations)
 int forward(winds_mlp *m, const float *X, int B, int Bp, bool keep_t, hipStream_t st) {
     Layer &l0 = m->L[0];
+    if (keep_t && m->fused) {
+        // the whole forward per 32 rows, leaving Z, AT and XT as below
+        FusedNet net{};
+        FusedTrain tr{};
+        net.nl = m->nl;
+        net.n_in = m->n_in;
+        net.n_out = m->n_out;
+        for (int i = 0; i < m->nl; ++i) {
+            Layer &l = m->L[i];
+            net.L[i] = FusedLayer{l.Wb.as<bf16>(), l.b.as<float>(), l.Kp, l.Np};
+            tr.Z[i] = l.Z.as<float>();
+            tr.AT[i] = l.AT.as<bf16>();
+        }
+        tr.XT = m->XT.as<bf16>();
+        tr.Bp = Bp;
+        mlp_forward_fused<true><<<(unsigned)(Bp / kFRows), kFThreads, 0, st>>>(net, X, B, nullptr,
+                                                                              tr);
+        MLP_HIP(hipGetLastError());
+        return WINDS_OK;
+    }
     if (keep_t)
         tile_pack<0><<<dim3((l0.Kp + 63) / 64, Bp / 64), 256, 0, st>>>(
             nullptr, X, nullptr, B, m->n_in, Bp, l0.Kp, m->Xb.as<bf16>(), m->XT.as<bf16>());
@@ -950,8 +1000,8 @@ int winds_mlp_forward(winds_mlp *m, const float *X, int32_t B, float *Y, void *s
         for (int i = 0; i < m->nl; ++i)
             net.L[i] = FusedLayer{m->L[i].Wb.as<bf16>(), m->L[i].b.as<float>(), m->L[i].Kp,
                                   m->L[i].Np};
-        mlp_forward_fused<<<(unsigned)((B + kFRows - 1) / kFRows), kFThreads, 0, st>>>(net, X, B,
-                                                                                     Y);
+        mlp_forward_fused<false><<<(unsigned)((B + kFRows - 1) / kFRows), kFThreads, 0, st>>>(
+            net, X, B, Y, FusedTrain{});
         MLP_HIP(hipGetLastError());
         return WINDS_OK;
     }

@@ -118,8 +118,9 @@ class MLP:
         return out
 
     def set_fused(self, on: bool) -> None:
-        """Inference in one kernel per 64 rows (True, the default for hidden
-        widths up to 256) or one GEMM per layer (False); the same results."""
+        """Forward passes (inference and training) in one kernel per 32 rows
+        (True, the default for hidden widths up to 256) or one GEMM per layer
+        (False); the same results."""
         _lib.check(self.L.winds_mlp_set_fused(self._m, 1 if on else 0))
 
     def gradients(self, X, T):
