@@ -383,6 +383,123 @@ __global__ __launch_bounds__(kFThreads) void mlp_forward_fused(FusedNet net, con
     }
 }
 
+// The backward pass's deltas for 32 rows in one workgroup: the output
+// layer's delta from the squared error (mse_grad's arithmetic), then layer
+// by layer dZ_prev = (D W) SiLU'(Z_prev) with the deltas D held in LDS as
+// BF16 and W^T's fragments read from device memory, as gemm_bf16 with
+// kEpiSiluBack computes them; each layer's FP32 delta (for the bias sums)
+// and its transposed BF16 copy (for the weight gradients) go to device
+// memory.  The loss adds one partial per workgroup atomically.
+struct FusedBack {
+    const float *Z[kFMaxLayers];  // pre-activations [Bp][Np]
+    float *dZ[kFMaxLayers];       // FP32 deltas [Bp][Np]
+    bf16 *DT[kFMaxLayers];        // BF16 deltas transposed [Np][Bp]
+    const bf16 *WT[kFMaxLayers];  // W^T [Kp][Np]
+    int Np[kFMaxLayers], Kp[kFMaxLayers];
+    const float *T;
+    float *loss;
+    int B, Bp, nl, n_out;
+};
+
+__global__ __launch_bounds__(kFThreads) void mlp_backward_fused(FusedBack fb) {
+    __shared__ __attribute__((aligned(16))) bf16 dl[2][kFRows][kFLdsRow];
+    __shared__ float part[kFThreads / 64];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int row0 = blockIdx.x * kFRows;
+    const bf16x8 zero = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int lo = fb.nl - 1;
+    // W^T fragments of layer i for this wave's column block, all at once
+    bf16x8 bw[kFSteps];
+    auto load_w = [&](int i) {
+        const int K = (fb.Np[i] + kSlice - 1) / kSlice * kSlice;
+        const bool on = w < fb.Kp[i] / 32;
+        const bf16 *row = fb.WT[i] + (int64_t)(w * 32 + (l & 31)) * fb.Np[i];
+#pragma unroll
+        for (int s = 0; s < kFSteps; ++s) {
+            const int k = 16 * s + 8 * (l >> 5);
+            bw[s] = (16 * s < K && on && k < fb.Np[i]) ? *(const bf16x8 *)(row + k) : zero;
+        }
+    };
+    if (lo >= 1) load_w(lo);
+    // the output layer's delta: 2 (y - t) / (B n_out) on real rows and
+    // outputs, zero elsewhere
+    {
+        const int Npo = fb.Np[lo];
+        const int K = (Npo + kSlice - 1) / kSlice * kSlice;
+        const float sc = 1.0f / ((float)fb.B * (float)fb.n_out);
+        float contrib = 0.0f;
+        for (int i = t; i < kFRows * K; i += kFThreads) {
+            const int r = i / K, c = i - r * K, b = row0 + r;
+            if (c >= Npo) {
+                dl[0][r][c] = 0;
+                continue;
+            }
+            float g = 0.0f;
+            if (b < fb.B && c < fb.n_out) {
+                const float rr = fb.Z[lo][(int64_t)b * Npo + c] - fb.T[(int64_t)b * fb.n_out + c];
+                g = 2.0f * rr * sc;
+                contrib += rr * rr * sc;
+            }
+            fb.dZ[lo][(int64_t)b * Npo + c] = g;
+            dl[0][r][c] = to_bf16(g);
+        }
+        if (fb.loss) {
+            for (int off = 32; off > 0; off >>= 1) contrib += __shfl_down(contrib, off);
+            if (l == 0) part[w] = contrib;
+        }
+        __syncthreads();
+        if (fb.loss && t == 0) {
+            float s = 0.0f;
+            for (int q = 0; q < kFThreads / 64; ++q) s += part[q];
+            if (s != 0.0f) atomicAdd(fb.loss, s);
+        }
+        store_transposed(dl[0], Npo, fb.DT[lo], fb.Bp, row0);
+    }
+    for (int i = lo; i >= 1; --i) {
+        const bf16(*in)[kFLdsRow] = dl[(lo - i) & 1];
+        bf16(*out)[kFLdsRow] = dl[(lo - i + 1) & 1];
+        const int K = (fb.Np[i] + kSlice - 1) / kSlice * kSlice;
+        const int Nout = fb.Kp[i];  // = the previous layer's Np
+        const bool on = w < Nout / 32;
+        f32x16 acc = {0};
+        if (on) {
+#pragma unroll
+            for (int s = 0; s < kFSteps; ++s) {
+                if (16 * s >= K) break;
+                const int k = 16 * s + 8 * (l >> 5);
+                const bf16x8 a = *(const bf16x8 *)&in[l & 31][k];
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    __builtin_bit_cast(bf16x8v, a), __builtin_bit_cast(bf16x8v, bw[s]), acc, 0, 0,
+                    0);
+            }
+        }
+        if (i > 1) load_w(i - 1);
+        if (on) {
+            const int col = w * 32 + (l & 31);
+            const float *Zp = fb.Z[i - 1];
+            float *dZp = fb.dZ[i - 1];
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int rl = 8 * (v >> 2) + 4 * (l >> 5) + (v & 3);
+                const int64_t o = (int64_t)(row0 + rl) * Nout + col;
+                const float z = Zp[o];
+                const float sg = sigm(z);
+                const float d = acc[v] * (sg * (1.0f + z * (1.0f - sg)));
+                dZp[o] = d;
+                out[rl][col] = to_bf16(d);
+            }
+        }
+        // the next product's slices past this width are zeros
+        const int Kn = (Nout + kSlice - 1) / kSlice * kSlice;
+        for (int q = t; q < kFRows * (Kn - Nout); q += kFThreads) {
+            const int r = q / (Kn - Nout), k = Nout + q % (Kn - Nout);
+            out[r][k] = 0;
+        }
+        __syncthreads();
+        store_transposed(out, Nout, fb.DT[i - 1], fb.Bp, row0);
+    }
+}
+
 // Row and column of flat index i in rows of N: 32-bit division whenever i
 // fits (every batch these tables hold; the 64-bit form costs ~4x the
 // instructions in these memory-bound elementwise kernels).
@@ -749,8 +866,31 @@ int forward(winds_mlp *m, const float *X, int B, int Bp, bool keep_t, hipStream_
     return WINDS_OK;
 }
 
+int weight_gradients(winds_mlp *m, int Bp, hipStream_t st);
+
 int backward(winds_mlp *m, const float *T, int B, int Bp, hipStream_t st, float *loss) {
     Layer &lo = m->L[m->nl - 1];
+    if (m->fused) {
+        FusedBack fb{};
+        for (int i = 0; i < m->nl; ++i) {
+            Layer &l = m->L[i];
+            fb.Z[i] = l.Z.as<float>();
+            fb.dZ[i] = l.dZ.as<float>();
+            fb.DT[i] = l.DT.as<bf16>();
+            fb.WT[i] = l.WT.as<bf16>();
+            fb.Np[i] = l.Np;
+            fb.Kp[i] = l.Kp;
+        }
+        fb.T = T;
+        fb.loss = loss;
+        fb.B = B;
+        fb.Bp = Bp;
+        fb.nl = m->nl;
+        fb.n_out = m->n_out;
+        mlp_backward_fused<<<(unsigned)(Bp / kFRows), kFThreads, 0, st>>>(fb);
+        MLP_HIP(hipGetLastError());
+        return weight_gradients(m, Bp, st);
+    }
     mse_grad<<<nblk((int64_t)Bp * lo.Np), 256, 0, st>>>(lo.Z.as<float>(), T, B, m->n_out, Bp, lo.Np,
                                                        lo.dZ.as<float>(), loss);
     MLP_HIP(hipGetLastError());
@@ -770,7 +910,13 @@ int backward(winds_mlp *m, const float *T, int B, int Bp, hipStream_t st, float 
         gemm<kEpiSiluBack>(st, l.Db.as<bf16>(), l.Np, l.WT.as<bf16>(), l.Np, Bp, l.Kp, l.Np, e);
         MLP_HIP(hipGetLastError());
     }
-    // then every layer's weight gradient in one launch, dW [Np][Kp] = dZ^T
+    return weight_gradients(m, Bp, st);
+}
+
+// Every layer's weight gradient in one launch and the bias partials in
+// another, from the deltas and activations the passes left.
+int weight_gradients(winds_mlp *m, int Bp, hipStream_t st) {
+    // every layer's weight gradient in one launch, dW [Np][Kp] = dZ^T
     // [Np][Bp] * X^T [Kp][Bp]^T as S partial slabs over the batch
     // (S = splits_of(Bp)), and the bias gradients as one partial per 64 rows
     const int S = splits_of(Bp);
