@@ -11,8 +11,11 @@
 // SiLU's derivative (backward) and writes the BF16 activations or deltas in
 // both [batch][feature] and [feature][batch] layouts, so that the forward
 // pass (X W^T), the input gradient (dZ W) and the weight gradient (dZ^T X)
-// all use this one kernel.  Inference runs the whole network in one kernel
-// per 32 rows instead (mlp_forward_fused), with bitwise the same results.
+// all use this one kernel.  Where the hidden width is at most 256 the
+// row-local passes run in one kernel per 32 rows instead, with bitwise the
+// same results: the forward (mlp_forward_fused, for inference and for
+// training) and the backward deltas (mlp_backward_fused); the weight
+// gradients stay GEMMs, every layer's in one launch (gemm_bf16_group).
 
 #include <hip/hip_runtime.h>
 
