@@ -34,7 +34,7 @@ def test_every_declared_symbol_is_exported():
     for n in march:
         assert hasattr(L, n), n
     mlp = header_functions("winds_mlp.h")
-    assert len(mlp) == 9
+    assert len(mlp) == 10
     for n in mlp:
         assert hasattr(L, n), n
     sur = header_functions("winds_surrogate.h")
