@@ -16,6 +16,7 @@ namespace winds {
 // FP32 arithmetic never meets r1 r2 + r1.r2 ~ 0 (0.5 plus rounding margin).
 constexpr double kMinNearLenFrac = 0.51;
 
+
 // Orders the wave's LDS accesses: LDS operations of one wave execute in
 // issue order; this keeps the compiler from moving them across each other.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -141,16 +142,45 @@ __device__ __forceinline__ RelPoint rel_point(double px, double py, double pz, d
     return r;
 }
 
-// The FP32 fields of a record as the pair arithmetic reads them, all as SGPR
-// operands.  (Moving the three components of L, which feed three operations
-// each, into VGPRs first paid off at 7 waves per SIMD; at 8 waves the moves
-// cost more than the SGPR operands: 2% slower.)
+// The FP32 fields of a record as the pair arithmetic reads them.
+//
+// Measured on gfx950 (8 waves per SIMD, independent chains): a wave64 FP32
+// instruction with VGPR, inline-constant or literal operands issues in ~2.3
+// shader clocks, one with an SGPR operand in ~4.1, FP64 adds and
+// conversions in ~4.2, v_rsq_f32 in ~9.8.  So fields read as SGPR operands
+// cost twice their issue time.  The cut-off core reads L and gamma (nine
+// uses per record) from a per-wave LDS copy of the tile instead, broadcast
+// into VGPRs (its evaluation 27.6 -> 25.3 ms at 133,920 x 263,427); the
+// Vatistas cores, whose records keep more values live, spill more with the
+// extra VGPRs and gain nothing, and keep every field in SGPRs.
 struct RecF {
     float lx, ly, lz, near2b, il2, g, core, near2;
 };
 
 __device__ __forceinline__ RecF rec_fields_1(const Seg32 &s) {
     return RecF{s.lx, s.ly, s.lz, s.near2b, s.il2, s.g, s.core, s.near2};
+}
+
+// L and gamma from the LDS copy (v = its float4), the rest from the record.
+__device__ __forceinline__ RecF rec_fields_v(const Seg32 &s, const float4 &v) {
+    return RecF{v.x, v.y, v.z, s.near2b, s.il2, v.w, s.core, s.near2};
+}
+
+// (lx, ly, lz, g) of the kSegTile records of tile kt copied into buf (one
+// float4 per record): lane l loads field (l & 3) of records (l >> 2) + 16 q.
+__device__ __forceinline__ void rec_stage(const Seg32 *__restrict__ s32, int64_t kt, float *buf) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const int c = lane & 3;
+    float v[kSegTile / 16];
+#pragma unroll
+    for (int q = 0; q < kSegTile / 16; ++q) {
+        const float *r = (const float *)(s32 + kt + 16 * q + (lane >> 2));
+        v[q] = r[c < 3 ? 6 + c : 11];  // lx, ly, lz at words 6-8, g at word 11
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < kSegTile / 16; ++q) buf[64 * q + lane] = v[q];
+    wave_lds_sync();
 }
 
 // Near pairs go to a per-wave list in LDS instead of a branch in the pair
@@ -163,7 +193,8 @@ __device__ __forceinline__ RecF rec_fields_1(const Seg32 &s) {
 // instruction), so the result is the same on every run.
 constexpr int kNearCap = 256;  // pairs per wave between evaluations
 
-struct NearLds {
+struct alignas(16) NearLds {
+    float rec[4 * kSegTile];  // the tile's (lx, ly, lz, g) (cut-off core)
     uint32_t pair[kNearCap];  // (k - k0) * 64 + lane
     double p[3][64];          // the wave's targets
     double acc[3][64];        // their FP64 near-pair sums
@@ -252,9 +283,9 @@ struct NearList {
 // fields f) added to (qx, qy, qz) in FP32, or to (ax, ay, az) by the FP64
 // reference arithmetic for a near pair.  Returns P - B for the next record
 // of the filament.
-// SECOND: the record is the second of a pair; when it continues the first
-// one's filament it forms P - B as r1 - L in FP32.
-template <int CORE, int SEM, bool CO, bool SECOND>
+// CHAINED32: when the record continues the filament of the record before
+// it forms P - B as r1 - L in FP32.
+template <int CORE, int SEM, bool CO, bool CHAINED32>
 __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz, const Seg32 &s,
                                                  const RecF &f, const Seg64 *__restrict__ q64,
                                                  int64_t k, RelPoint a, const Fp32Params &P,
@@ -270,12 +301,12 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
         mc = __builtin_amdgcn_ballot_w64(!(a.s >= s.near2a));
     }
     const float Lx = f.lx, Ly = f.ly, Lz = f.lz;
-    // P - B: in FP64 and rounded once, or (SECOND, chained) as r1 - L in
-    // FP32, where r1 was rounded from FP64 one record before.  The pairs that
-    // reach the FP32 arithmetic have |r2| >= 0.51|L|, so the subtraction adds
-    // a few roundings of |r2| at most.
+    // P - B: in FP64 and rounded once, or (CHAINED32, chained) as r1 - L in
+    // FP32, where r1 was rounded from FP64 at most three records before.  The
+    // pairs that reach the FP32 arithmetic have |r2| >= 0.51|L|, so each
+    // subtraction adds a few roundings of |r2| at most.
     RelPoint b;
-    if (SECOND && chained) {
+    if (CHAINED32 && chained) {
         b.x = a.x - Lx;
         b.y = a.y - Ly;
         b.z = a.z - Lz;
@@ -383,6 +414,35 @@ __device__ __forceinline__ Seg32 load_record(const Seg32 *__restrict__ s32, int6
     return s;
 }
 
+// load_record with the record's LDS float4 v (read before the scalar loads)
+// in the same wait: one lgkmcnt wait covers both.
+__device__ __forceinline__ Seg32 load_record_v(const Seg32 *__restrict__ s32, int64_t k,
+                                               const float4 &v) {
+    Seg32 s = s32[k];
+    asm volatile("" : "+s"(s.flags)
+                 : "s"(s.bx), "s"(s.by), "s"(s.bz), "s"(s.lx), "s"(s.ly), "s"(s.lz),
+                   "s"(s.near2b), "s"(s.il2), "s"(s.g), "s"(s.core), "s"(s.near2),
+                   "s"(s.near2a), "v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w));
+    return s;
+}
+
+// Record k (slot j of the tile) and the fields its arithmetic reads: from
+// the scalar record, or with LF L and gamma from the tile's LDS copy buf.
+template <bool LF>
+__device__ __forceinline__ Seg32 load_rec(const Seg32 *__restrict__ s32, int64_t k,
+                                          const float *buf, int j, RecF &f) {
+    if constexpr (LF) {
+        const float4 v = ((const float4 *)buf)[j];
+        const Seg32 s = load_record_v(s32, k, v);
+        f = rec_fields_v(s, v);
+        return s;
+    } else {
+        const Seg32 s = load_record(s32, k);
+        f = rec_fields_1(s);
+        return s;
+    }
+}
+
 // Sum of the terms of segments [k0, k1) (prepared records, k1 - k0 a
 // multiple of kSegTile) at target (px, py, pz).  s64 + shift is the FP64
 // record of prepared record k.  Every lane of the wave must walk the same
@@ -393,9 +453,13 @@ __device__ __forceinline__ Seg32 load_record(const Seg32 *__restrict__ s32, int6
 // SEM:  0 MEX semantics (near pairs by biot_p2p), 1 BiotSavart.m semantics
 //       (near pairs by the BiotSavart.m arithmetic with zeroing).
 // CO:   apply the far cut-off r1 > co && r2 > co (BiotSavart.m:111).
-// PAIR32: the second record of each pair forms P - B as r1 - L in FP32
-// when it continues the first one's filament.
-template <int CORE, int SEM, bool CO, bool PAIR32 = true>
+// CHAIN32: records go in groups of four; the first forms P - B in FP64,
+// the other three as r1 - L in FP32 when they continue the filament (one
+// FP64 point per four records: 29.0 -> 28.7 ms for Vatistas-2 and, with
+// the cut-off core's LDS fields, 27.6 -> 25.3 ms at 133,920 x 263,427;
+// largest per-target error 1.8e-7 -> 2.7e-7 against FP64).
+// LF:   L and gamma from a per-wave LDS copy of the tile (see RecF).
+template <int CORE, int SEM, bool CO, bool CHAIN32 = true, bool LF = (CORE == 0)>
 __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                                             const Seg32 *__restrict__ s32,
                                             const Seg64 *__restrict__ s64, int64_t shift,
@@ -415,32 +479,51 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
         e = rel_point(px, py, pz, s64[k0 + shift].ax, s64[k0 + shift].ay, s64[k0 + shift].az);
         mc = __builtin_amdgcn_ballot_w64(!(e.s >= s32[k0].near2a));
     }
+    const float *buf = nlds->rec;
+    if constexpr (LF) {
+        if (k0 < k1) rec_stage(s32, k0, nlds->rec);
+    }
     // Each iteration loads its own records: at 8 waves per SIMD the other
     // waves hide the scalar-cache latency (loading the next iteration's
     // records early helped only the general Vatistas order while it ran 4
     // waves, and made the cut-off 6% slower).
     for (int64_t kt = k0; kt < k1; kt += kSegTile) {
         float qx = 0.f, qy = 0.f, qz = 0.f;
-        // two records per iteration: the carried point alternates between
+        // four records per iteration: the carried point alternates between
         // two sets of registers instead of being copied
 #pragma unroll 1
-        for (int j = 0; j < kSegTile; j += 2) {
+        for (int j = 0; j < kSegTile; j += 4) {
             const int64_t k = kt + j;
-            const Seg32 sa = load_record(s32, k);
+            RecF fa, fb, fc, fd;
+            const Seg32 sa = load_rec<LF>(s32, k, buf, j, fa);
             RelPoint f = e;
             if (!(sa.flags & kSegSkip))
-                f = fp32c_record<CORE, SEM, CO, false>(px, py, pz, sa, rec_fields_1(sa), s64 + k + shift,
-                                                k, e, P, qx, qy, qz, mc, nl);
-            const Seg32 sb = load_record(s32, k + 1);
+                f = fp32c_record<CORE, SEM, CO, false>(px, py, pz, sa, fa, s64 + k + shift, k, e,
+                                                       P, qx, qy, qz, mc, nl);
+            const Seg32 sb = load_rec<LF>(s32, k + 1, buf, j + 1, fb);
             if (!(sb.flags & kSegSkip))
-                e = fp32c_record<CORE, SEM, CO, PAIR32>(px, py, pz, sb, rec_fields_1(sb),
-                                                      s64 + k + 1 + shift, k + 1, f, P, qx, qy,
-                                                      qz, mc, nl);
+                e = fp32c_record<CORE, SEM, CO, CHAIN32>(px, py, pz, sb, fb, s64 + k + 1 + shift,
+                                                         k + 1, f, P, qx, qy, qz, mc, nl);
             else
                 e = f;
-            // flushed here, between record pairs, where little else is live
+            // flushed between record pairs, where little else is live
             // (inside the record it pushed the loop to 89 VGPRs, 5 waves)
             if (nl.n > kNearCap - 128) nl.flush(P);
+            const Seg32 sc = load_rec<LF>(s32, k + 2, buf, j + 2, fc);
+            f = e;
+            if (!(sc.flags & kSegSkip))
+                f = fp32c_record<CORE, SEM, CO, CHAIN32>(px, py, pz, sc, fc, s64 + k + 2 + shift,
+                                                         k + 2, e, P, qx, qy, qz, mc, nl);
+            const Seg32 sd = load_rec<LF>(s32, k + 3, buf, j + 3, fd);
+            if (!(sd.flags & kSegSkip))
+                e = fp32c_record<CORE, SEM, CO, CHAIN32>(px, py, pz, sd, fd, s64 + k + 3 + shift,
+                                                         k + 3, f, P, qx, qy, qz, mc, nl);
+            else
+                e = f;
+            if (nl.n > kNearCap - 128) nl.flush(P);
+        }
+        if constexpr (LF) {
+            if (kt + kSegTile < k1) rec_stage(s32, kt + kSegTile, nlds->rec);
         }
         two_sum_acc(hx, lx, qx);
         two_sum_acc(hy, ly, qy);
