@@ -210,12 +210,13 @@ def mlp_weights(n_in: int, hidden: int, layers: int, n_out: int) -> int:
     return n_in * hidden + (layers - 1) * hidden * hidden + hidden * n_out
 
 
-def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int = 4096):
-    """Far-wake network: direct-sum training data from seeded synthetic wakes
-    (TSR U[4,10], yaw U[0,30] deg, split at 144 ages = 2 revolutions) plus,
-    when given, direct sums from the engine's own march; training with the
-    engine's BF16 MFMA kernels; held-out error and inference rate.
-    Returns (report, network, in_norm, out_norm)."""
+def surrogate_bench(n_points: int, train_steps: int, march_set=None, batch: int = 4096):
+    """Far-wake network: direct-sum training data from the engine's own
+    marches at n_points stratified operating points (TSR 4-10 x yaw 0-30 deg,
+    1,440-age wakes, split at 144 ages = 2 revolutions), training with the
+    engine's BF16 MFMA kernels; its error on march_set, the benchmark march's
+    own direct sums (TSR 7, no yaw: an operating point it was not trained
+    on); inference rate.  Returns (report, network, in_norm, out_norm)."""
     import torch
 
     import winds
@@ -224,55 +225,33 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
 
     o = march_opts(winds.PREC_FP32C)
     sc = S.MarchSurrogate()
-    ts = S.build_training_set(n_samples, o, seed=7, split=sc.split, target_ages=sc.target_ages,
-                              log=log)
-    log(f"surrogate data: {ts.wakes} synthetic wakes, {ts.F.shape[0]} samples, "
+    pts = S.operating_points(n_points, seed=17)
+    ts = S.marched_training_set(pts, o, max_ages=1440, split=sc.split,
+                                target_ages=sc.target_ages, every=8, log=log)
+    log(f"surrogate data: {len(pts)} marched wakes, {ts.wakes} snapshots, {ts.F.shape[0]} samples, "
         f"{ts.interactions / 1e9:.1f} G interactions, {ts.seconds:.1f} s")
-    held = S.build_training_set(50_000, o, seed=8, split=sc.split, target_ages=sc.target_ages)
     rows = 31 * sc.target_ages * 3 + 30 * 3
     net = MLP(S.NIN, MLP_HIDDEN, MLP_LAYERS, S.NOUT, max(batch, rows), seed=7)
-    pre = {}
-    if march_set is not None:
-        def pretrained(net_, inn, onn):
-            # the network after the synthetic phase only: how it generalises
-            # to held-out synthetic wakes and to the march it has not seen
-            Hn_, _ = S.normalise(held.F, None, inn, onn)
-            Fm_, _ = S.normalise(march_set.F, None, inn, onn)
-            Yh, Ym = S.predict(net_, Hn_, onn), S.predict(net_, Fm_, onn)
-            pre.update({
-                "normwise_err_synthetic_heldout": S.normwise_error(Yh, held.Y),
-                "rms_err_synthetic_heldout": S.rms_error(Yh, held.Y),
-                "normwise_err_march_samples": S.normwise_error(Ym, march_set.Y),
-                "rms_err_march_samples": S.rms_error(Ym, march_set.Y)})
-            log(f"  after the synthetic phase: {pre}")
-        in_norm, out_norm, (s1, s2), (l1, l2) = S.train_two_phase(
-            net, ts, march_set, train_steps, batch, log=log, between=pretrained)
-        secs = s1 + s2
-        loss = {"synthetic_phase": l1, "march_phase": l2}
-    else:
-        in_norm, out_norm = S.fit_norm(ts.F, ts.Y)
-        Fn, Yn = S.normalise(ts.F, ts.Y, in_norm, out_norm)
-        secs, loss = S.train(net, Fn, Yn, steps=train_steps, batch=batch, lr=1e-3, log=log)
-        del Fn, Yn
-    Hn, _ = S.normalise(held.F, None, in_norm, out_norm)
+    in_norm, out_norm = S.fit_norm(ts.F, ts.Y)
+    Fn, Yn = S.normalise(ts.F, ts.Y, in_norm, out_norm)
+    secs, loss = S.train(net, Fn, Yn, steps=train_steps, batch=batch, lr=1e-3, log=log)
+    del Fn, Yn
     W = mlp_weights(S.NIN, MLP_HIDDEN, MLP_LAYERS, S.NOUT)
     # training: forward 2W, weight gradients 2W, input gradients 2W (less the first layer)
     train_flop = (6 * W - 2 * S.NIN * MLP_HIDDEN) * batch
     train_tflops = train_flop * train_steps / secs / 1e12
     Ft, _ = S.normalise(ts.F[:200_000].contiguous(), None, in_norm, out_norm)
-    Yt, Yh = S.predict(net, Ft, out_norm), S.predict(net, Hn, out_norm)
-    err_train = S.normwise_error(Yt, ts.Y[:200_000])
-    err_held = S.normwise_error(Yh, held.Y)
-    rms_train, rms_held = S.rms_error(Yt, ts.Y[:200_000]), S.rms_error(Yh, held.Y)
+    Yt = S.predict(net, Ft, out_norm)
+    err_train, rms_train = S.normwise_error(Yt, ts.Y[:200_000]), S.rms_error(Yt, ts.Y[:200_000])
     err_march = rms_march = None
     if march_set is not None:
         Fm, _ = S.normalise(march_set.F, None, in_norm, out_norm)
         Ym = S.predict(net, Fm, out_norm)
         err_march, rms_march = S.normwise_error(Ym, march_set.Y), S.rms_error(Ym, march_set.Y)
     log(f"surrogate training: {train_steps} steps x {batch}: {secs:.1f} s "
-        f"({train_tflops:.1f} TFLOP/s), loss {loss}, normwise error synthetic train "
-        f"{err_train:.3e} held-out {err_held:.3e} march samples {err_march}; rms "
-        f"{rms_train:.3e} / {rms_held:.3e} / {rms_march}")
+        f"({train_tflops:.1f} TFLOP/s), loss {loss:.3e}, normwise error train "
+        f"{err_train:.3e}, held-out operating point {err_march}; rms {rms_train:.3e} / "
+        f"{rms_march}")
     # inference rate on max_batch rows
     B = net.max_batch
     Xb = Ft[:B].contiguous()
@@ -297,23 +276,25 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
                    f"activations, FP32 accumulation (v_mfma_f32_32x32x16_bf16)",
         "parameters": net.num_params(),
         "split_age": sc.split, "target_ages": sc.target_ages,
-        "training": "half the steps on synthetic wakes, half on the march's own samples",
-        "synthetic_samples": int(ts.F.shape[0]), "synthetic_wakes": ts.wakes,
-        "synthetic_data_G_interactions": round(ts.interactions / 1e9, 2),
-        "synthetic_data_s": round(ts.seconds, 2),
-        "march_samples": int(march_set.F.shape[0]) if march_set is not None else 0,
-        "march_snapshots": march_set.wakes if march_set is not None else 0,
+        "training": ("direct sums of the engine's own marches at stratified operating "
+                     "points (TSR 4-10 x yaw 0-30 deg; 1,440-age wakes developed by the "
+                     "direct-far march, 2 revolutions sampled every 8 steps); the benchmark "
+                     "march's operating point (TSR 7, yaw 0) is not among them"),
+        "operating_points": [[round(t, 3), round(y, 2)] for t, y in pts],
+        "samples": int(ts.F.shape[0]), "snapshots": ts.wakes,
+        "data_G_interactions": round(ts.interactions / 1e9, 2),
+        "data_s": round(ts.seconds, 2),
         "train_steps": train_steps, "batch": batch, "train_s": round(secs, 2),
         "train_samples_per_s": round(train_steps * batch / secs, 1),
         "train_tflops": round(train_tflops, 2),
         "final_loss": loss,
         "error_metrics": "normwise = max |error| / max |label|; rms = rms |error| / rms |label| "
                          "(3-vectors of far-wake velocity over U)",
-        "normwise_err_synthetic_train": err_train, "normwise_err_synthetic_heldout": err_held,
-        "normwise_err_march_samples": err_march,
-        "rms_err_synthetic_train": rms_train, "rms_err_synthetic_heldout": rms_held,
-        "rms_err_march_samples": rms_march,
-        "after_synthetic_phase_only": pre or None,
+        "normwise_err_train": err_train, "rms_err_train": rms_train,
+        "heldout": "the benchmark march's own direct sums (TSR 7, yaw 0)",
+        "heldout_samples": int(march_set.F.shape[0]) if march_set is not None else 0,
+        "heldout_snapshots": march_set.wakes if march_set is not None else 0,
+        "normwise_err_heldout": err_march, "rms_err_heldout": rms_march,
         "inference_rows": B, "inference_ms": round(fwd_ms, 4),
         "inference_samples_per_s": round(inf_rate, 1),
         "inference_tflops": round(inf_tflops, 2),
@@ -324,19 +305,22 @@ def surrogate_bench(n_samples: int, train_steps: int, march_set=None, batch: int
     return rep, net, in_norm, out_norm
 
 
-def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train_steps: int,
+def march_bench(revs: int, dev_revs: int, surrogate: bool, n_points: int, train_steps: int,
                 check_every: int = 72, fp64_revs: int = 2, total_steps: int = 3600):
     """Free-wake time march of a 3-blade, 31-node rotor at 5 deg steps with
     PCC (2 wake evaluations per step), an FP64 Kutta-Joukowski solve and a
     1,440-age wake (~10 rotor diameters):
     (1) a cold start of dev_revs revolutions with FP32 compensated wake sums
-        (direct-sum snapshots of its far wake feed the surrogate's training);
+        (then one revolution of direct-sum snapshots of its full-length far
+        wake measures the network, trained on other operating points, here);
     (2) step time at the full length from that state: FP32, FP64, surrogate
         (network), surrogate with the direct far sum instead of the network;
-    (3) `revs` more revolutions from that state: FP32 and surrogate, and the
-        first fp64_revs of them in FP64 as the reference for both; the
-        surrogate march goes on to total_steps steps in all (the workload's
-        3,600), its later revolutions set beside the FP32 march's last ones.
+    (3) `revs` more revolutions from that state: FP32, the surrogate (with
+        its online training at each check) and the surrogate with the network
+        as trained on the other operating points (no online training), and
+        the first fp64_revs of them in FP64 as the reference; the surrogate
+        march goes on to total_steps steps in all (the workload's 3,600), its
+        later revolutions set beside the FP32 march's last ones.
     Returns (march report, surrogate report)."""
     import numpy as np
 
@@ -359,17 +343,8 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
     # (1) develop
     t0 = time.perf_counter()
     m = March(tb, 1440, o32, integrator="pcc")
-    first = max(0, dev_steps - 10 * per_rev)
-    m.step(first)
+    m.step(dev_steps)
     m.sync()
-    log(f"march cold start: {first} steps in {time.perf_counter() - t0:.1f} s")
-    msamp = None
-    if surrogate:
-        msamp = S.march_training_set(m, o32, steps=dev_steps - first, every=12,
-                                     split=sc.split, target_ages=sc.target_ages, log=log)
-    else:
-        m.step(dev_steps - first)
-        m.sync()
     dev_s = time.perf_counter() - t0
     log(f"march cold start {dev_revs} revs: {dev_s:.1f} s")
     inf = m.info()
@@ -377,19 +352,26 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
     A, Ag = inf["ages"], inf["ages_solved"]
     gs, gt = m.state_circulation(Ag)
     state = (inf["p"], nodes, gs[:, :, :A - 1], gt[:, :, :A - 2])
-    del m
     out["develop_s_fp32c"] = round(dev_s, 2)
+    msamp = None
+    if surrogate:
+        # the network's held-out data: direct sums of this march's far wake
+        # at full length, one revolution from the developed state
+        msamp = S.march_training_set(m, o32, steps=per_rev, every=8, split=sc.split,
+                                     target_ages=sc.target_ages, log=log)
+    del m
 
     surr_rep, net_args = None, None
     if surrogate:
-        surr_rep, net, in_norm, out_norm = surrogate_bench(n_samples, train_steps, msamp)
+        surr_rep, net, in_norm, out_norm = surrogate_bench(n_points, train_steps, msamp)
         net_args = (net, in_norm, out_norm)
 
-    def fork(prec, surr_mode=None):
+    def fork(prec, surr_mode=None, train_iters=40):
         f = March(tb, 1440, march_opts(prec), integrator="pcc")
         f.set_state(*state)
         if surr_mode is not None:
-            cfg = S.MarchSurrogate(mode=surr_mode, check_every=check_every)
+            cfg = S.MarchSurrogate(mode=surr_mode, check_every=check_every,
+                                   train_iters=train_iters)
             if surr_mode == 0:
                 f.set_surrogate(cfg, *net_args)
             else:
@@ -444,13 +426,14 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
         f"{out['short_wake_step_ms']}")
     # (3) revs more revolutions from the developed state
     hist = {}
-    plan = [("fp32c", winds.PREC_FP32C, None, steps)]
+    plan = [("fp32c", winds.PREC_FP32C, None, steps, 40)]
     if surrogate:
-        plan += [("surrogate", winds.PREC_FP32C, 0, surr_steps)]
-    plan += [("fp64", winds.PREC_FP64, None, fp64_revs * per_rev)]
-    for name, prec, mode, nst in plan:
+        plan += [("surrogate", winds.PREC_FP32C, 0, surr_steps, 40),
+                 ("surrogate_pretrained_only", winds.PREC_FP32C, 0, steps, 0)]
+    plan += [("fp64", winds.PREC_FP64, None, fp64_revs * per_rev, 40)]
+    for name, prec, mode, nst, iters in plan:
         t0 = time.perf_counter()
-        f = fork(prec, mode)
+        f = fork(prec, mode, iters)
         done = 0
         while done < nst:
             k = min(per_rev, nst - done)
@@ -463,7 +446,7 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
         out[f"continue_s_{name}"] = round(el, 2)
         log(f"march {nst} more steps {name}: {el:.1f} s")
         if mode is not None:
-            out["surrogate_stats_continue"] = f.surr_stats()
+            out[f"{name}_stats_continue"] = f.surr_stats()
         del f
 
     def errs(a, b, n):
@@ -483,6 +466,12 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_samples: int, train
     if surrogate:
         out["surrogate_vs_fp64"] = errs(hist["surrogate"], b64, n64)
         out["surrogate_vs_fp32c"] = errs(hist["surrogate"], hist["fp32c"], steps)
+        out["surrogate_pretrained_only_vs_fp64"] = errs(hist["surrogate_pretrained_only"], b64,
+                                                        n64)
+        out["surrogate_pretrained_only_vs_fp32c"] = errs(hist["surrogate_pretrained_only"],
+                                                         hist["fp32c"], steps)
+        log(f"surrogate without online training vs fp32c over {revs} revs: "
+            f"{out['surrogate_pretrained_only_vs_fp32c']}")
         log(f"surrogate vs fp64 over {fp64_revs} revs: {out['surrogate_vs_fp64']}; "
             f"vs fp32c over {revs} revs: {out['surrogate_vs_fp32c']}")
         # the surrogate's last revolution against the FP32 march's last one
@@ -522,7 +511,9 @@ def main():
     ap.add_argument("--develop-revs", type=int, default=20,
                     help="cold-start revolutions before the full-length measurements")
     ap.add_argument("--no-surrogate", action="store_true", help="skip the far-wake network")
-    ap.add_argument("--surrogate-samples", type=int, default=2_000_000)
+    ap.add_argument("--surrogate-points", type=int, default=16,
+                    help="operating points whose marched wakes train the far-wake network "
+                         "(a multiple of 4)")
     ap.add_argument("--surrogate-steps", type=int, default=8000)
     ap.add_argument("--check-every", type=int, default=72,
                     help="surrogate steps between direct-sum checks")
@@ -665,7 +656,7 @@ def main():
     march, surr_rep = None, None
     if not args.no_march:
         march, surr_rep = march_bench(args.march_revs, args.develop_revs, not args.no_surrogate,
-                                      args.surrogate_samples, args.surrogate_steps,
+                                      args.surrogate_points, args.surrogate_steps,
                                       args.check_every, args.fp64_revs)
 
     achieved = inter * FLOP_VATISTAS2 / (ms * 1e-3) / 1e12

@@ -288,3 +288,59 @@ def test_full_rotor_surrogate_within_one_percent(need_gpu):
     assert st["checks"] == 4 and st["network_steps"] + st["direct_far_steps"] == steps - 4
     assert st["network_steps"] >= steps // 2
     assert err <= 0.01
+
+
+def test_network_from_other_operating_points_serves_a_new_one(need_gpu):
+    """A network trained only on the engine's own marches at 16 stratified
+    operating points (TSR 4-10 x yaw 0-30 deg, 3 blades x 31 nodes, 432-age
+    wakes, split at 144), used with no online training in a march at TSR
+    6.5, yaw 15 deg, which is not among them: its error on that march's own
+    direct sums is bounded (measured RMS 2-3.5% at four such points), every
+    check passes at 1%, the network serves every step between checks, and
+    thrust, torque and power stay within 1% of the plain FP32 march over 4
+    revolutions (measured 0.07-0.34%)."""
+    from winds import surrogate as S
+    from winds.march import March
+    from winds.mlp import MLP
+    o32 = winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, precision=winds.PREC_FP32C,
+                     d=1e-4)
+    per_rev, A = 72, 432
+    pts = S.operating_points(16, seed=17)
+    assert min(abs(t - 6.5) + abs(y - 15.0) / 5 for t, y in pts) > 1.0
+    data = S.marched_training_set(pts, o32, max_ages=A, direct_far=False, every=8)
+    in_norm, out_norm = S.fit_norm(data.F, data.Y)
+    Fn, Yn = S.normalise(data.F, data.Y, in_norm, out_norm)
+    net = MLP(S.NIN, 256, 4, S.NOUT, 10240, seed=7)
+    secs, loss = S.train(net, Fn, Yn, steps=6000, batch=4096)
+    del Fn, Yn
+    rot = cases.tables(cases.synthetic_rotor(nb=3, nst=31, tsr=6.5, yaw_deg=15.0,
+                                             n_times=A + 6 * per_rev))
+    m = March(rot, A, o32, integrator="pcc")
+    m.step(A)
+    inf = m.info()
+    nodes, _, _ = m.state()
+    gs, gt = m.state_circulation(inf["ages_solved"])
+    state = (inf["p"], nodes, gs[:, :, :inf["ages"] - 1], gt[:, :, :inf["ages"] - 2])
+    held = S.march_training_set(m, o32, steps=per_rev, every=12, split=144, target_ages=72)
+    del m
+    Hn, _ = S.normalise(held.F, None, in_norm, out_norm)
+    Yp = S.predict(net, Hn, out_norm)
+    rms, nw = S.rms_error(Yp, held.Y), S.normwise_error(Yp, held.Y)
+    a, b = March(rot, A, o32, integrator="pcc"), March(rot, A, o32, integrator="pcc")
+    for f in (a, b):
+        f.set_state(*state)
+    b.set_surrogate(S.MarchSurrogate(mode=0, check_every=per_rev, train_iters=0), net, in_norm,
+                    out_norm)
+    steps = 4 * per_rev
+    for f in (a, b):
+        f.step(steps)
+        f.sync()
+    err = rel_perf_err(b.perf(), a.perf())
+    st = b.surr_stats()
+    flush_print(f"[other operating points] {data.F.shape[0]} samples from {len(pts)} marches "
+                f"({data.seconds:.1f} s), loss {loss:.2e}; held-out rms {rms:.3f} normwise "
+                f"{nw:.3f}; T/Q/P err {err:.2e}; {st}")
+    assert rms <= 0.1
+    assert st["adam_steps"] == 0 and st["checks"] == 4 and st["failed_checks"] == 0
+    assert st["network_steps"] == steps - 4
+    assert err <= 0.01

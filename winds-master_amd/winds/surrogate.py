@@ -3,10 +3,13 @@ configuration (include/winds_surrogate.h, include/winds_march.h).
 
 The network maps a query point of the near wake or the lifting line plus a
 descriptor of the far wake (every segment with an endpoint of age >= split)
-to the far wake's induced velocity there.  Training data come from seeded
-synthetic wakes (winds.gen.rotor_wake) over TSR 4-10 and yaw 0-30 deg; the
-labels are direct sums of the engine's FP32-compensated kernel over the far
-segments, and the inputs are built by the same device code the march uses.
+to the far wake's induced velocity there.  Training data come from the
+engine's own free-wake marches at seeded operating points over TSR 4-10 and
+yaw 0-30 deg (marched_training_set), or from prescribed rigid helices
+(build_training_set, which do not teach the rolled-up, expanding wake of a
+march); the labels are direct sums of the engine's FP32-compensated kernel
+over the far segments, and the inputs are built by the same device code the
+march uses.
 """
 from __future__ import annotations
 
@@ -262,6 +265,62 @@ def march_training_set(m, opts, steps: int, every: int, split: int, target_ages:
         snaps += 1
         if log and snaps % 10 == 0:
             log(f"  march snapshots {snaps} (step {done})")
+    torch.cuda.synchronize()
+    return TrainingSet(torch.cat(Fs).contiguous(), torch.cat(Ys).contiguous(), snaps, inter,
+                       time.perf_counter() - t0)
+
+
+def operating_points(n: int, seed: int = 17, tsr_range=(4.0, 10.0),
+                     yaw_range=(0.0, 30.0)) -> list:
+    """n seeded (TSR, yaw in degrees) pairs, stratified: the TSR x yaw box
+    cut into 4 x (n // 4) cells (n a multiple of 4), one uniform point in
+    each, so that every region of the box has training wakes."""
+    if n % 4:
+        raise ValueError("operating_points: n must be a multiple of 4")
+    rng = np.random.default_rng(seed)
+    nt, ny = 4, n // 4
+    (t0, t1), (y0, y1) = tsr_range, yaw_range
+    return [(t0 + (t1 - t0) * (i + rng.uniform()) / nt, y0 + (y1 - y0) * (j + rng.uniform()) / ny)
+            for i in range(nt) for j in range(ny)]
+
+
+def marched_training_set(points, opts, max_ages: int = 1440, split: int = 144,
+                         target_ages: int = 72, sample_revs: int = 2, every: int = 6,
+                         nb: int = 3, nst: int = 31, U: float = 11.4, direct_far: bool = True,
+                         log=None) -> TrainingSet:
+    """Direct-sum samples of the engine's own marches, one per operating
+    point (tsr, yaw_deg) of `points`: a synthetic rotor (winds.cases.
+    synthetic_rotor, designed for that TSR) marched with PCC until its wake
+    has max_ages ages, then sample_revs more revolutions with one snapshot of
+    the far wake's velocity at the near targets every `every` steps.  With
+    direct_far the wake is developed by the surrogate's direct-far mode (near
+    sum every step, older nodes' velocities refreshed once per revolution:
+    ~10x cheaper per step at 1,440 ages); the snapshots are full direct sums
+    either way."""
+    import torch
+    from . import cases
+    from .march import March
+    per_rev = 72  # the synthetic rotor turns 5 degrees per step
+    Fs, Ys, inter, snaps = [], [], 0, 0
+    t0 = time.perf_counter()
+    for i, (tsr, yaw) in enumerate(points):
+        tb = cases.tables(cases.synthetic_rotor(nb=nb, nst=nst, tsr=tsr, U=U, yaw_deg=yaw,
+                                                n_times=max_ages + (sample_revs + 1) * per_rev))
+        m = March(tb, max_ages, opts, integrator="pcc")
+        if direct_far:
+            m.set_surrogate(MarchSurrogate(mode=1, split=split, target_ages=target_ages,
+                                           check_every=per_rev))
+        m.step(max_ages)
+        ts = march_training_set(m, opts, steps=sample_revs * per_rev, every=every, split=split,
+                                target_ages=target_ages)
+        del m
+        Fs.append(ts.F)
+        Ys.append(ts.Y)
+        inter += ts.interactions
+        snaps += ts.wakes
+        if log:
+            log(f"  marched wake {i + 1}/{len(points)} (TSR {tsr:.2f}, yaw {yaw:.1f} deg): "
+                f"{ts.F.shape[0]} samples, {time.perf_counter() - t0:.1f} s")
     torch.cuda.synchronize()
     return TrainingSet(torch.cat(Fs).contiguous(), torch.cat(Ys).contiguous(), snaps, inter,
                        time.perf_counter() - t0)
