@@ -453,12 +453,17 @@ __device__ __forceinline__ Seg32 load_rec(const Seg32 *__restrict__ s32, int64_t
 // SEM:  0 MEX semantics (near pairs by biot_p2p), 1 BiotSavart.m semantics
 //       (near pairs by the BiotSavart.m arithmetic with zeroing).
 // CO:   apply the far cut-off r1 > co && r2 > co (BiotSavart.m:111).
-// CHAIN32: records go in groups of four; the first forms P - B in FP64,
-// the other three as r1 - L in FP32 when they continue the filament (one
-// FP64 point per four records: 29.0 -> 28.7 ms for Vatistas-2 and, with
-// the cut-off core's LDS fields, 27.6 -> 25.3 ms at 133,920 x 263,427;
-// largest per-target error 1.8e-7 -> 2.7e-7 against FP64).
+// CHAIN32: records go in groups of kGroup; the first forms P - B in FP64,
+// the others as r1 - L in FP32 when they continue the filament (one FP64
+// point per four records instead of two: Vatistas-2 29.3 -> 29.0 ms and,
+// with the cut-off core's LDS fields, cut-off 27.7 -> 26.0 ms at 133,920 x
+// 263,427; largest per-target error 1.8e-7 -> 2.7e-7 against FP64).
 // LF:   L and gamma from a per-wave LDS copy of the tile (see RecF).
+// Records per FP64 end point.  Longer chains are faster but carry more
+// FP32 rounding into r: on the 64-wake batch at full size the largest
+// per-target error was 3.0e-7 with 4, 6.1e-7 with 8 (normwise 1.3e-7) and
+// 2.8e-6 with 16, for 1.4% and 2.6% less time on the 133,920 x 263,427 wake.
+constexpr int kGroup = 4;
 template <int CORE, int SEM, bool CO, bool CHAIN32 = true, bool LF = (CORE == 0)>
 __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                                             const Seg32 *__restrict__ s32,
@@ -489,38 +494,36 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
     // waves, and made the cut-off 6% slower).
     for (int64_t kt = k0; kt < k1; kt += kSegTile) {
         float qx = 0.f, qy = 0.f, qz = 0.f;
-        // four records per iteration: the carried point alternates between
+        // kGroup records per iteration: the carried point alternates between
         // two sets of registers instead of being copied
 #pragma unroll 1
-        for (int j = 0; j < kSegTile; j += 4) {
-            const int64_t k = kt + j;
-            RecF fa, fb, fc, fd;
-            const Seg32 sa = load_rec<LF>(s32, k, buf, j, fa);
-            RelPoint f = e;
-            if (!(sa.flags & kSegSkip))
-                f = fp32c_record<CORE, SEM, CO, false>(px, py, pz, sa, fa, s64 + k + shift, k, e,
-                                                       P, qx, qy, qz, mc, nl);
-            const Seg32 sb = load_rec<LF>(s32, k + 1, buf, j + 1, fb);
-            if (!(sb.flags & kSegSkip))
-                e = fp32c_record<CORE, SEM, CO, CHAIN32>(px, py, pz, sb, fb, s64 + k + 1 + shift,
-                                                         k + 1, f, P, qx, qy, qz, mc, nl);
-            else
-                e = f;
-            // flushed between record pairs, where little else is live
-            // (inside the record it pushed the loop to 89 VGPRs, 5 waves)
-            if (nl.n > kNearCap - 128) nl.flush(P);
-            const Seg32 sc = load_rec<LF>(s32, k + 2, buf, j + 2, fc);
-            f = e;
-            if (!(sc.flags & kSegSkip))
-                f = fp32c_record<CORE, SEM, CO, CHAIN32>(px, py, pz, sc, fc, s64 + k + 2 + shift,
-                                                         k + 2, e, P, qx, qy, qz, mc, nl);
-            const Seg32 sd = load_rec<LF>(s32, k + 3, buf, j + 3, fd);
-            if (!(sd.flags & kSegSkip))
-                e = fp32c_record<CORE, SEM, CO, CHAIN32>(px, py, pz, sd, fd, s64 + k + 3 + shift,
-                                                         k + 3, f, P, qx, qy, qz, mc, nl);
-            else
-                e = f;
-            if (nl.n > kNearCap - 128) nl.flush(P);
+        for (int j = 0; j < kSegTile; j += kGroup) {
+#pragma unroll
+            for (int q = 0; q < kGroup; q += 2) {
+                const int64_t k = kt + j + q;
+                RecF fa, fb;
+                const Seg32 sa = load_rec<LF>(s32, k, buf, j + q, fa);
+                RelPoint f = e;
+                if (!(sa.flags & kSegSkip)) {
+                    if (q == 0)
+                        f = fp32c_record<CORE, SEM, CO, false>(px, py, pz, sa, fa, s64 + k + shift,
+                                                               k, e, P, qx, qy, qz, mc, nl);
+                    else
+                        f = fp32c_record<CORE, SEM, CO, CHAIN32>(px, py, pz, sa, fa,
+                                                                 s64 + k + shift, k, e, P, qx, qy,
+                                                                 qz, mc, nl);
+                }
+                const Seg32 sb = load_rec<LF>(s32, k + 1, buf, j + q + 1, fb);
+                if (!(sb.flags & kSegSkip))
+                    e = fp32c_record<CORE, SEM, CO, CHAIN32>(px, py, pz, sb, fb,
+                                                             s64 + k + 1 + shift, k + 1, f, P, qx,
+                                                             qy, qz, mc, nl);
+                else
+                    e = f;
+                // flushed between record pairs, where little else is live
+                // (inside the record it pushed the loop to 89 VGPRs, 5 waves)
+                if (nl.n > kNearCap - 128) nl.flush(P);
+            }
         }
         if constexpr (LF) {
             if (kt + kSegTile < k1) rec_stage(s32, kt + kSegTile, nlds->rec);
