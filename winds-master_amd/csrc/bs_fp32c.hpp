@@ -283,6 +283,19 @@ struct NearList {
 // fields f) added to (qx, qy, qz) in FP32, or to (ax, ay, az) by the FP64
 // reference arithmetic for a near pair.  Returns P - B for the next record
 // of the filament.
+// Far cut-off masks of the wave's lanes at the carried point (CO kernels):
+// band, |r|^2 within FP32 rounding of co^2 (the pair goes to FP64); far,
+// not |r|^2 <= co^2 (beyond co, or NaN).  Carried like the end-point mask,
+// so each point is tested once for the two segments that share it.
+struct CoMask {
+    uint64_t band, far;
+};
+
+__device__ __forceinline__ CoMask co_mask(float s2, const Fp32Params &P) {
+    return CoMask{__builtin_amdgcn_ballot_w64(!(fabsf(s2 - P.co2) > P.co2_tol)),
+                  __builtin_amdgcn_ballot_w64(!(s2 <= P.co2))};
+}
+
 // CHAINED32: when the record continues the filament of the record before
 // it forms P - B as r1 - L in FP32.
 template <int CORE, int SEM, bool CO, bool CHAINED32>
@@ -290,7 +303,7 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
                                                  const RecF &f, const Seg64 *__restrict__ q64,
                                                  int64_t k, RelPoint a, const Fp32Params &P,
                                                  float &qx, float &qy, float &qz, uint64_t &mc,
-                                                 NearList<CORE, SEM> &nl) {
+                                                 CoMask &cm, NearList<CORE, SEM> &nl) {
     // mc: the wave's lanes passing the end-point test at A.  A chained record
     // takes it from the record before (whose B threshold covers this
     // record's), the others test their own start point.  Kept as a lane mask
@@ -299,6 +312,7 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     if (!chained) {
         a = rel_point(px, py, pz, q64->ax, q64->ay, q64->az);
         mc = __builtin_amdgcn_ballot_w64(!(a.s >= s.near2a));
+        if constexpr (CO) cm = co_mask(a.s, P);
     }
     const float Lx = f.lx, Ly = f.ly, Lz = f.lz;
     // P - B: in FP64 and rounded once, or (CHAINED32, chained) as r1 - L in
@@ -336,18 +350,23 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     // combined per-lane condition would go through a VGPR and a compare)
     const uint64_t m3 = __builtin_amdgcn_ballot_w64(n3);
     uint64_t m = __builtin_amdgcn_ballot_w64(n1) & (mc | m3);
+    // The far cut-off (zero when r1 > co and r2 > co) is a step: a pair with
+    // r1 or r2 within FP32 rounding of co could fall on the other side of it
+    // than in FP64 and drop or keep a whole term, so such pairs take the FP64
+    // branch, which decides as the reference does.
+    CoMask cb{0, 0};
     if constexpr (CO) {
-        // The far cut-off (zero when r1 > co and r2 > co) is a step: a pair
-        // with r1 or r2 within FP32 rounding of co could fall on the other
-        // side of it than in FP64 and drop or keep a whole term, so such
-        // pairs take the FP64 branch, which decides as the reference does.
-        const bool ca = !(fabsf(a.s - P.co2) > P.co2_tol);
-        const bool cb = !(fabsf(b.s - P.co2) > P.co2_tol);
-        m |= __builtin_amdgcn_ballot_w64(ca) | __builtin_amdgcn_ballot_w64(cb);
+        cb = co_mask(b.s, P);
+        m |= cm.band | cb.band;
     }
     if constexpr (CORE == 3) m |= __builtin_amdgcn_ballot_w64(h2 >= P.h2_big);
     const bool near = __builtin_amdgcn_inverse_ballot_w64(m);
+    // zeroed: near pairs (their terms come from the FP64 branch) and, with
+    // the far cut-off, pairs with both ends beyond co
+    uint64_t zm = m;
+    if constexpr (CO) zm |= cm.far & cb.far;
     mc = m3;
+    if constexpr (CO) cm = cb;
     // r1 r2 + r1.r2: from the carried lengths (one product per point, where
     // s1 s2 / (r1 r2) costs two per record), or, for the cut-off core, whose
     // d|L|^2 term needs 1/(r1 r2) anyway, from that
@@ -391,9 +410,7 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
         rq = fmaf(rq, fmaf(-qa, rq, 1.0f), rq);
         ubar = gn * (h2 * root * rq);
     }
-    bool zero = near;
-    if constexpr (CO) zero = zero | (!(a.s <= P.co2) & !(b.s <= P.co2));
-    ubar = zero ? 0.0f : ubar;
+    ubar = __builtin_amdgcn_inverse_ballot_w64(zm) ? 0.0f : ubar;
     qx = fmaf(ubar, cx, qx);
     qy = fmaf(ubar, cy, qy);
     qz = fmaf(ubar, cz, qz);
@@ -480,9 +497,11 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
     // range's first record forms it from its own start point
     RelPoint e{0.f, 0.f, 0.f, 1.f, 1.f, 1.f};
     uint64_t mc = 0;  // lanes passing the end-point near test at that point
+    CoMask cm{0, 0};  // the far cut-off masks at that point (CO)
     if (k0 < k1 && !(s32[k0].flags & kSegSkip)) {
         e = rel_point(px, py, pz, s64[k0 + shift].ax, s64[k0 + shift].ay, s64[k0 + shift].az);
         mc = __builtin_amdgcn_ballot_w64(!(e.s >= s32[k0].near2a));
+        if constexpr (CO) cm = co_mask(e.s, P);
     }
     const float *buf = nlds->rec;
     if constexpr (LF) {
@@ -507,17 +526,17 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                 if (!(sa.flags & kSegSkip)) {
                     if (q == 0)
                         f = fp32c_record<CORE, SEM, CO, false>(px, py, pz, sa, fa, s64 + k + shift,
-                                                               k, e, P, qx, qy, qz, mc, nl);
+                                                               k, e, P, qx, qy, qz, mc, cm, nl);
                     else
                         f = fp32c_record<CORE, SEM, CO, CHAIN32>(px, py, pz, sa, fa,
                                                                  s64 + k + shift, k, e, P, qx, qy,
-                                                                 qz, mc, nl);
+                                                                 qz, mc, cm, nl);
                 }
                 const Seg32 sb = load_rec<LF>(s32, k + 1, buf, j + q + 1, fb);
                 if (!(sb.flags & kSegSkip))
                     e = fp32c_record<CORE, SEM, CO, CHAIN32>(px, py, pz, sb, fb,
                                                              s64 + k + 1 + shift, k + 1, f, P, qx,
-                                                             qy, qz, mc, nl);
+                                                             qy, qz, mc, cm, nl);
                 else
                     e = f;
                 // flushed between record pairs, where little else is live
