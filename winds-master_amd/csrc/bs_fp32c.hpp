@@ -144,15 +144,16 @@ __device__ __forceinline__ RelPoint rel_point(double px, double py, double pz, d
 
 // The FP32 fields of a record as the pair arithmetic reads them.
 //
-// Measured on gfx950 (8 waves per SIMD, independent chains): a wave64 FP32
-// instruction with VGPR, inline-constant or literal operands issues in ~2.3
-// shader clocks, one with an SGPR operand in ~4.1, FP64 adds and
-// conversions in ~4.2, v_rsq_f32 in ~9.8.  So fields read as SGPR operands
-// cost twice their issue time.  The cut-off core reads L and gamma (nine
-// uses per record) from a per-wave LDS copy of the tile instead, broadcast
-// into VGPRs (its evaluation 27.6 -> 25.3 ms at 133,920 x 263,427); the
-// Vatistas cores, whose records keep more values live, spill more with the
-// extra VGPRs and gain nothing, and keep every field in SGPRs.
+// Measured on gfx950 with independent chains at 8 waves per SIMD: a wave64
+// FP32 instruction with VGPR, inline-constant or literal operands issues in
+// ~2.3 shader clocks, one with an SGPR operand in ~4.1, FP64 adds and
+// conversions in ~4.2, v_rsq_f32 in ~9.8.  In the kernel's mixed stream
+// those costs do not simply add: for the Vatistas cores, copying L into
+// VGPRs once per record made the evaluation 3% slower, and reading L and
+// gamma from LDS took 5% fewer cycles but ran at a 9% lower clock; they keep
+// every field in SGPRs.  The cut-off core, with fewer values live per
+// record, reads L and gamma (nine uses per record) from a per-wave LDS copy
+// of the tile, broadcast into VGPRs: 27.7 -> 26.0 ms at 133,920 x 263,427.
 struct RecF {
     float lx, ly, lz, near2b, il2, g, core, near2;
 };
@@ -168,6 +169,7 @@ __device__ __forceinline__ RecF rec_fields_v(const Seg32 &s, const float4 &v) {
 
 // (lx, ly, lz, g) of the kSegTile records of tile kt copied into buf (one
 // float4 per record): lane l loads field (l & 3) of records (l >> 2) + 16 q.
+static_assert(kSegTile % 16 == 0, "rec_stage: 16 records per 64 lanes");
 __device__ __forceinline__ void rec_stage(const Seg32 *__restrict__ s32, int64_t kt, float *buf) {
     const int lane = (int)(threadIdx.x & 63u);
     const int c = lane & 3;
