@@ -632,6 +632,20 @@ def main():
     torch.cuda.synchronize()
     ms_c = c0.elapsed_time(c1) / nc
     rate_c = inter / (ms_c * 1e-3) / 1e9
+    # its error against the FP64 MEX arithmetic (biot_p2p, bitwise the CPU oracle's)
+    uc = out.cpu().numpy()
+    n_fix_c = ev.guard_count()
+    o64c = winds.opts(core=winds.CORE_CUTOFF, semantics=winds.SEM_MEX, precision=winds.PREC_FP64,
+                      d=1e-4)
+    refc = ev.eval(tg, sg, o64c).cpu().numpy()
+    nrm_c = np.linalg.norm(refc, axis=0)
+    dif_c = np.linalg.norm(uc - refc, axis=0)
+    mc_ = nrm_c >= 1e-6 * nrm_c.max()
+    ec = dif_c[mc_] / nrm_c[mc_]
+    err_c = {"max_rel": float(ec.max()), "rms_rel": float(np.sqrt(np.mean(ec * ec))),
+             "normwise": float(dif_c.max() / nrm_c.max()), "vs": "FP64 engine (biot_p2p)",
+             "guard_fp64_targets": n_fix_c}
+    log(f"cut-off: {ms_c:.3f} ms, {rate_c:.1f} G int/s, error vs fp64 {err_c}")
 
     # the drop-in call itself: host arrays in MATLAB layout, the same segments
     # and targets, Vatistas-2 with BiotSavart.m semantics, transfers included
@@ -683,7 +697,10 @@ def main():
         "error": err,
         "cutoff_mex": {"value": round(rate_c, 2), "unit": "G interactions/s",
                        "ms_per_step": round(ms_c, 4),
-                       "tflops": round(inter * FLOP_CUTOFF / (ms_c * 1e-3) / 1e12, 2)},
+                       "tflops": round(inter * FLOP_CUTOFF / (ms_c * 1e-3) / 1e12, 2),
+                       "frac_of_fp32_peak": round(inter * FLOP_CUTOFF / (ms_c * 1e-3) / 1e12
+                                                  / FP32_PEAK_TFLOPS, 4),
+                       "error": err_c},
         "fp64_ms_per_step": round(t64 * 1e3, 2),
         "ms_per_step_guard_off": round(ms_noguard, 4),
         "drop_in_call_ms_host_arrays": round(abi_ms, 2),
