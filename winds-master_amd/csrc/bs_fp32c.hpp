@@ -153,7 +153,7 @@ __device__ __forceinline__ RelPoint rel_point(double px, double py, double pz, d
 // gamma from LDS took 5% fewer cycles but ran at a 9% lower clock; they keep
 // every field in SGPRs.  The cut-off core, with fewer values live per
 // record, reads L and gamma (nine uses per record) from a per-wave LDS copy
-// of the tile, broadcast into VGPRs: 27.7 -> 26.0 ms at 133,920 x 263,427.
+// of the tile, broadcast into VGPRs: 27.7 -> 26.6 ms at 133,920 x 263,427.
 struct RecF {
     float lx, ly, lz, near2b, il2, g, core, near2;
 };
@@ -318,7 +318,7 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     }
     const float Lx = f.lx, Ly = f.ly, Lz = f.lz;
     // P - B: in FP64 and rounded once, or (CHAINED32, chained) as r1 - L in
-    // FP32, where r1 was rounded from FP64 at most three records before.  The
+    // FP32, where r1 was rounded from FP64 one record before.  The
     // pairs that reach the FP32 arithmetic have |r2| >= 0.51|L|, so each
     // subtraction adds a few roundings of |r2| at most.
     RelPoint b;
@@ -473,16 +473,16 @@ __device__ __forceinline__ Seg32 load_rec(const Seg32 *__restrict__ s32, int64_t
 //       (near pairs by the BiotSavart.m arithmetic with zeroing).
 // CO:   apply the far cut-off r1 > co && r2 > co (BiotSavart.m:111).
 // CHAIN32: records go in groups of kGroup; the first forms P - B in FP64,
-// the others as r1 - L in FP32 when they continue the filament (one FP64
-// point per four records instead of two: Vatistas-2 29.3 -> 29.0 ms and,
-// with the cut-off core's LDS fields, cut-off 27.7 -> 26.0 ms at 133,920 x
-// 263,427; largest per-target error 1.8e-7 -> 2.7e-7 against FP64).
+// the others as r1 - L in FP32 when they continue the filament.
 // LF:   L and gamma from a per-wave LDS copy of the tile (see RecF).
+//
 // Records per FP64 end point.  Longer chains are faster but carry more
-// FP32 rounding into r: on the 64-wake batch at full size the largest
-// per-target error was 3.0e-7 with 4, 6.1e-7 with 8 (normwise 1.3e-7) and
-// 2.8e-6 with 16, for 1.4% and 2.6% less time on the 133,920 x 263,427 wake.
-constexpr int kGroup = 4;
+// FP32 rounding into r, and the cancellation guard's error estimate assumes
+// the rounding of two: with 4 the largest per-target error on 32 seeded
+// filament soups (random polylines, targets on and near them) grew from
+// 4.1e-7 to 9.8e-7 for 1.7% less time on the 133,920 x 263,427 wake; with
+// 8 and 16 the 64-wake batch's grew from 3.0e-7 to 6.1e-7 and 2.8e-6.
+constexpr int kGroup = 2;
 template <int CORE, int SEM, bool CO, bool CHAIN32 = true, bool LF = (CORE == 0)>
 __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                                             const Seg32 *__restrict__ s32,
