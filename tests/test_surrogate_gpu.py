@@ -4,8 +4,9 @@ Inputs are checked against a numpy restatement of their definition
 (include/winds_surrogate.h); the march is checked (a) to reduce exactly to
 the plain march when every step is a check, (b) to stay within 1% of the
 plain march's thrust and power with the direct far sum and the held far-node
-velocities, and (c) to gain over dropping the far wake when the network
-supplies the far wake's velocity."""
+velocities, (c) to gain over dropping the far wake when the network
+supplies the far wake's velocity, and (d) to serve an operating point the
+network was not trained on, with no online training, within 1%."""
 import math
 
 import numpy as np
