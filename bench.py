@@ -19,6 +19,7 @@ import sys
 import time
 from pathlib import Path
 
+T_START = time.monotonic()
 ROOT = Path(__file__).resolve().parent
 for _p in (str(ROOT), str(ROOT / "winds-master_amd")):
     if _p not in sys.path:
@@ -38,6 +39,27 @@ FLOP_CUTOFF = 47           # cut-off (MEX) form
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+class Budget:
+    """Wall-clock budget of the whole run (seconds from process start).  A
+    stage whose estimated time does not fit in what is left is skipped and
+    named under "skipped" in the output line, so the line is printed in time
+    whatever the host's speed."""
+
+    def __init__(self, seconds: float):
+        self.seconds = seconds
+        self.skipped = []
+
+    def left(self) -> float:
+        return self.seconds - (time.monotonic() - T_START)
+
+    def fits(self, stage: str, est_s: float, reserve_s: float = 20.0) -> bool:
+        ok = est_s + reserve_s <= self.left()
+        if not ok:
+            self.skipped.append(f"{stage} (estimated {est_s:.0f} s, {self.left():.0f} s left)")
+            log(f"skipping {stage}: estimated {est_s:.0f} s, {self.left():.0f} s left")
+        return ok
 
 
 def host_info() -> dict:
@@ -306,7 +328,8 @@ def surrogate_bench(n_points: int, train_steps: int, march_set=None, batch: int 
 
 
 def march_bench(revs: int, dev_revs: int, surrogate: bool, n_points: int, train_steps: int,
-                check_every: int = 72, fp64_revs: int = 2, total_steps: int = 3600):
+                check_every: int = 72, fp64_revs: int = 2, total_steps: int = 3600,
+                budget: "Budget | None" = None):
     """Free-wake time march of a 3-blade, 31-node rotor at 5 deg steps with
     PCC (2 wake evaluations per step), an FP64 Kutta-Joukowski solve and a
     1,440-age wake (~10 rotor diameters):
@@ -321,6 +344,7 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_points: int, train_
         the first fp64_revs of them in FP64 as the reference; the surrogate
         march goes on to total_steps steps in all (the workload's 3,600), its
         later revolutions set beside the FP32 march's last ones.
+    Stages that do not fit in the budget are skipped (the budget names them).
     Returns (march report, surrogate report)."""
     import numpy as np
 
@@ -353,6 +377,12 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_points: int, train_
     gs, gt = m.state_circulation(Ag)
     state = (inf["p"], nodes, gs[:, :, :A - 1], gt[:, :, :A - 2])
     out["develop_s_fp32c"] = round(dev_s, 2)
+    # the network's data: one 1,440-step march per operating point in the
+    # direct-far mode, ~0.23 of the FP32 cold start's time each
+    if surrogate and budget is not None and not budget.fits(
+            "far-wake network (training data, training, surrogate marches)",
+            n_points * dev_s * 0.3 + 20):
+        surrogate, surr_steps = False, 0
     msamp = None
     if surrogate:
         # the network's held-out data: direct sums of this march's far wake
@@ -432,6 +462,10 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_points: int, train_
                  ("surrogate_pretrained_only", winds.PREC_FP32C, 0, steps, 0)]
     plan += [("fp64", winds.PREC_FP64, None, fp64_revs * per_rev, 40)]
     for name, prec, mode, nst, iters in plan:
+        ms_step = out[f"step_ms_{'surrogate' if mode is not None else name}"]
+        if budget is not None and not budget.fits(f"continued march {name} ({nst} steps)",
+                                                  nst * ms_step * 1.2e-3 + 3):
+            continue
         t0 = time.perf_counter()
         f = fork(prec, mode, iters)
         done = 0
@@ -456,24 +490,25 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_points: int, train_
         return {"thrust_max_rel": rel(1), "power_max_rel": rel(3), "thrust_mean_rel": meanr(1),
                 "power_mean_rel": meanr(3), "steps": int(n)}
 
-    n64 = fp64_revs * per_rev
-    b64 = hist["fp64"]
-    out["fp32c_vs_fp64"] = errs(hist["fp32c"], b64, n64)
-    out["thrust_N_mean_fp64"] = float(np.mean(b64[:, 1]))
-    out["power_W_mean_fp64"] = float(np.mean(b64[:, 3]))
-    out["kj_iterations_mean_fp64"] = float(np.mean(b64[:, 4]))
-    log(f"fp32c vs fp64 over {fp64_revs} revs at full length: {out['fp32c_vs_fp64']}")
-    if surrogate:
-        out["surrogate_vs_fp64"] = errs(hist["surrogate"], b64, n64)
-        out["surrogate_vs_fp32c"] = errs(hist["surrogate"], hist["fp32c"], steps)
-        out["surrogate_pretrained_only_vs_fp64"] = errs(hist["surrogate_pretrained_only"], b64,
-                                                        n64)
-        out["surrogate_pretrained_only_vs_fp32c"] = errs(hist["surrogate_pretrained_only"],
-                                                         hist["fp32c"], steps)
-        log(f"surrogate without online training vs fp32c over {revs} revs: "
-            f"{out['surrogate_pretrained_only_vs_fp32c']}")
-        log(f"surrogate vs fp64 over {fp64_revs} revs: {out['surrogate_vs_fp64']}; "
-            f"vs fp32c over {revs} revs: {out['surrogate_vs_fp32c']}")
+    n64 = fp64_revs * per_rev if "fp64" in hist else 0
+    b64 = hist.get("fp64")
+    if b64 is not None:
+        out["thrust_N_mean_fp64"] = float(np.mean(b64[:, 1]))
+        out["power_W_mean_fp64"] = float(np.mean(b64[:, 3]))
+        out["kj_iterations_mean_fp64"] = float(np.mean(b64[:, 4]))
+        if "fp32c" in hist:
+            out["fp32c_vs_fp64"] = errs(hist["fp32c"], b64, n64)
+            log(f"fp32c vs fp64 over {fp64_revs} revs at full length: {out['fp32c_vs_fp64']}")
+    for name in ("surrogate", "surrogate_pretrained_only"):
+        if name not in hist:
+            continue
+        if b64 is not None:
+            out[f"{name}_vs_fp64"] = errs(hist[name], b64, n64)
+            log(f"{name} vs fp64 over {fp64_revs} revs: {out[f'{name}_vs_fp64']}")
+        if "fp32c" in hist:
+            out[f"{name}_vs_fp32c"] = errs(hist[name], hist["fp32c"], steps)
+            log(f"{name} vs fp32c over {revs} revs: {out[f'{name}_vs_fp32c']}")
+    if "surrogate" in hist and "fp32c" in hist:
         # the surrogate's last revolution against the FP32 march's last one
         # (the rotor is periodic: same azimuths, same inflow)
         hs, hf = hist["surrogate"], hist["fp32c"]
@@ -486,14 +521,18 @@ def march_bench(revs: int, dev_revs: int, surrogate: bool, n_points: int, train_
                                     / abs(np.mean(last_f[:, 3])))}
         log(f"surrogate revolution {surr_steps // per_rev} vs fp32c revolution {revs}: "
             f"{out['surrogate_last_rev_vs_fp32c_last_rev']}")
+    ran = lambda name, n: n if name in hist else 0
     out.update({"revs": revs, "steps": steps,
-                "thrust_N_mean_fp32c": float(np.mean(hist["fp32c"][:, 1])),
-                "power_W_mean_fp32c": float(np.mean(hist["fp32c"][:, 3])),
-                "config5_steps_covered": (f"{dev_steps + steps} of 3600 steps in FP32 "
-                                          f"({dev_steps} cold start + {steps} continued), "
-                                          f"{dev_steps + surr_steps} with the surrogate "
-                                          f"({dev_steps} FP32 cold start + {surr_steps}), "
+                "config5_steps_covered": (f"{dev_steps + ran('fp32c', steps)} of 3600 steps in "
+                                          f"FP32 ({dev_steps} cold start + "
+                                          f"{ran('fp32c', steps)} continued), "
+                                          f"{dev_steps + ran('surrogate', surr_steps)} with the "
+                                          f"surrogate ({dev_steps} FP32 cold start + "
+                                          f"{ran('surrogate', surr_steps)}), "
                                           f"{n64} in FP64 from the developed 1,440-age state")})
+    if "fp32c" in hist:
+        out["thrust_N_mean_fp32c"] = float(np.mean(hist["fp32c"][:, 1]))
+        out["power_W_mean_fp32c"] = float(np.mean(hist["fp32c"][:, 3]))
     return out, surr_rep
 
 
@@ -521,7 +560,11 @@ def main():
                     help="also run one plain 2 GiB device-to-device copy (reference point for "
                          "the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--ages", type=int, default=1440)
+    ap.add_argument("--time-budget", type=float, default=540.0,
+                    help="seconds from process start; later stages that would not finish "
+                         "within it are skipped and listed under \"skipped\"")
     args = ap.parse_args()
+    budget = Budget(args.time_budget)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the engine "
               "is single-GPU in this build", file=sys.stderr)
@@ -666,12 +709,22 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    batch = None if args.no_batch else batch_bench(o32, o64, args.steps)
+    cb = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cb = cpu_baseline(wake)
+        # same core and arithmetic as the CPU path: the GPU cut-off rate
+        cb["gpu_cutoff_over_cpu_all_cores"] = round(rate_c / cb["value"], 1)
+
+    batch = None
+    if not args.no_batch and budget.fits("64-wake batch", 60):
+        batch = batch_bench(o32, o64, args.steps)
     march, surr_rep = None, None
-    if not args.no_march:
+    # cold start of the march: ~20 ms per step at 1,440 ages and fewer below
+    if not args.no_march and budget.fits("free-wake march", args.develop_revs * 72 * 0.03 + 30):
         march, surr_rep = march_bench(args.march_revs, args.develop_revs, not args.no_surrogate,
                                       args.surrogate_points, args.surrogate_steps,
-                                      args.check_every, args.fp64_revs)
+                                      args.check_every, args.fp64_revs, budget=budget)
 
     achieved = inter * FLOP_VATISTAS2 / (ms * 1e-3) / 1e12
     line = {
@@ -719,12 +772,10 @@ def main():
             "flop_per_interaction": FLOP_VATISTAS2,
         },
     }
-    if not args.no_cpu:
-        log("cpu baseline ...")
-        cb = cpu_baseline(wake)
-        # same core and arithmetic as the CPU path: the GPU cut-off rate
-        cb["gpu_cutoff_over_cpu_all_cores"] = round(rate_c / cb["value"], 1)
+    if cb is not None:
         line["cpu_baseline"] = cb
+    line["skipped"] = budget.skipped
+    line["run_s"] = round(time.monotonic() - T_START, 1)
     print(json.dumps(line), flush=True)
 
 
