@@ -33,7 +33,7 @@ FLOP_VATISTAS2 = 59        # algorithmic FLOPs per interaction, Vatistas n=2 for
 # HBM bytes per launch of bs_fp32c<2,1,false> on this workload: FETCH_SIZE /
 # 0.500 + WRITE_SIZE / 1.000 (rocprofv3 --pmc, factors from the plain
 # streaming kernel bs_reduce_split), profiles/bs_fp32c_vatistas2_single_wake_133920x263427.md
-TRAFFIC_BYTES = 1_193_400 * 1024
+TRAFFIC_BYTES = 1_179_900 * 1024
 FLOP_CUTOFF = 47           # cut-off (MEX) form
 
 

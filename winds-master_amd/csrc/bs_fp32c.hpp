@@ -311,7 +311,7 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     // record's), the others test their own start point.  Kept as a lane mask
     // in SGPRs: a carried per-lane bool is rebuilt from a VGPR each record.
     const bool chained = s.flags & kSegChained;
-    if (!chained) {
+    if (__builtin_expect(!chained, 0)) {
         a = rel_point(px, py, pz, q64->ax, q64->ay, q64->az);
         mc = __builtin_amdgcn_ballot_w64(!(a.s >= s.near2a));
         if constexpr (CO) cm = co_mask(a.s, P);
@@ -322,7 +322,7 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     // pairs that reach the FP32 arithmetic have |r2| >= 0.51|L|, so each
     // subtraction adds a few roundings of |r2| at most.
     RelPoint b;
-    if (CHAINED32 && chained) {
+    if (__builtin_expect(CHAINED32 && chained, 1)) {
         b.x = a.x - Lx;
         b.y = a.y - Ly;
         b.z = a.z - Lz;
@@ -416,7 +416,7 @@ __device__ __forceinline__ RelPoint fp32c_record(double px, double py, double pz
     qx = fmaf(ubar, cx, qx);
     qy = fmaf(ubar, cy, qy);
     qz = fmaf(ubar, cz, qz);
-    if (m) nl.add(m, near, k);
+    if (__builtin_expect(m != 0, 0)) nl.add(m, near, k);
     return b;
 }
 
@@ -476,6 +476,12 @@ __device__ __forceinline__ Seg32 load_rec(const Seg32 *__restrict__ s32, int64_t
 // the others as r1 - L in FP32 when they continue the filament.
 // LF:   L and gamma from a per-wave LDS copy of the tile (see RecF).
 //
+// The branch hints (__builtin_expect) here and in fp32c_record lay the
+// common path -- a record that continues the filament, no near lane, no
+// flush -- out to fall through, with the rare blocks out of line: taken
+// branches cost the wave issue slots (29.19 -> 28.37 ms on the Vatistas-2
+// wake of 133,920 targets x 263,427 segments, same results).
+//
 // Records per FP64 end point.  Longer chains are faster but carry more
 // FP32 rounding into r, and the cancellation guard's error estimate assumes
 // the rounding of two: with 4 the largest per-target error on 32 seeded
@@ -525,7 +531,7 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                 RecF fa, fb;
                 const Seg32 sa = load_rec<LF>(s32, k, buf, j + q, fa);
                 RelPoint f = e;
-                if (!(sa.flags & kSegSkip)) {
+                if (__builtin_expect(!(sa.flags & kSegSkip), 1)) {
                     if (q == 0)
                         f = fp32c_record<CORE, SEM, CO, false>(px, py, pz, sa, fa, s64 + k + shift,
                                                                k, e, P, qx, qy, qz, mc, cm, nl);
@@ -535,7 +541,7 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                                                                  qz, mc, cm, nl);
                 }
                 const Seg32 sb = load_rec<LF>(s32, k + 1, buf, j + q + 1, fb);
-                if (!(sb.flags & kSegSkip))
+                if (__builtin_expect(!(sb.flags & kSegSkip), 1))
                     e = fp32c_record<CORE, SEM, CO, CHAIN32>(px, py, pz, sb, fb,
                                                              s64 + k + 1 + shift, k + 1, f, P, qx,
                                                              qy, qz, mc, cm, nl);
@@ -543,7 +549,7 @@ __device__ __forceinline__ void fp32c_range(double px, double py, double pz,
                     e = f;
                 // flushed between record pairs, where little else is live
                 // (inside the record it pushed the loop to 89 VGPRs, 5 waves)
-                if (nl.n > kNearCap - 128) nl.flush(P);
+                if (__builtin_expect(nl.n > kNearCap - 128, 0)) nl.flush(P);
             }
         }
         if constexpr (LF) {
